@@ -1,0 +1,196 @@
+"""Benchmark: 3D diffusion (Poisson) operator apply on one MI355X.
+
+Headline (`value`): FP64 diffusion apply v = A u, p = 4 (P = 5 GLL nodes,
+Q = 6 Gauss points per direction), structured hex mesh of ~8M DoFs
+(n = 50, 8,120,601 DoFs), perturbed trilinear geometry, u ~ U[-1, 1]; in
+GDoF/s = 1e-9 * DoFs * applies / second.  One step = one CeedOperatorApply
+(zero v, then the fused E^T B^T D B E kernels over all element colours), with
+inputs resident in HBM.
+
+Also reported: a sweep over p = 1..8 at the same size in FP64 and FP32, the
+roofline of the fused kernel (algorithmic bytes per apply / kernel time,
+against 8 TB/s), and a CPU baseline: the reference's /cpu/self/avx/blocked
+(built from its sources, oracle/_ref) timed on this host, one process per core.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--p 4] [--dofs 8e6] [--no-sweep] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, vendor datasheet
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector/matrix, vendor datasheet
+
+
+def alg_bytes(N: int, E: int, P: int, Q: int, ncomp: int = 1, qcomp: int = 6, s: int = 8) -> int:
+    """Read u + write v + read q_data + read int32 offsets (each stream counted once)."""
+    return s * ncomp * N * 2 + s * qcomp * E * Q ** 3 + 4 * E * P ** 3
+
+
+def alg_flops(E: int, P: int, Q: int, ncomp: int = 1) -> int:
+    """Sum-factorised interp + collocated grad, forward and transpose, plus 15 flops/qpt of Poisson3DApply."""
+    return E * (2 * (2 * (P ** 3 * Q + P ** 2 * Q ** 2 + P * Q ** 3) + 6 * Q ** 4 * ncomp) + 15 * Q ** 3 + 2 * Q ** 3)
+
+
+def time_gpu(fn, steps: int, warmup: int):
+    """Average ms per call over `steps`, by HIP events on torch's current stream (where the kernels run) and by wall clock."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    s.record()
+    for _ in range(steps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    return s.elapsed_time(e) / steps, wall
+
+
+def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0):
+    import torch
+
+    from libceed_amd import ops
+    from libceed_amd.mesh import hex_mesh, mesh_size_for_dofs
+
+    n = mesh_size_for_dofs(p, dofs)
+    m = hex_mesh(n, p, seed=seed)
+    P, Q = p + 1, p + 2
+    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q, dtype=dtype)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    u = (torch.rand(m.num_nodes, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dtype)
+    v = torch.zeros_like(u)
+    ms_apply, wall = time_gpu(lambda: op.apply(u, v), steps, warmup)
+    ms_kernel, _ = time_gpu(lambda: op.apply_add(u, v), steps, warmup)
+    s = 8 if dtype == torch.float64 else 4
+    B = alg_bytes(m.num_nodes, m.num_elem, P, Q, s=s)
+    F = alg_flops(m.num_elem, P, Q)
+    res = {
+        "p": p, "n": n, "dofs": m.num_nodes, "elements": m.num_elem, "colors": len(op.colors()),
+        "ms_per_apply": ms_apply, "wall_ms_per_apply": wall, "gdofs": m.num_nodes / (ms_apply * 1e-3) / 1e9,
+        "kernel_ms": ms_kernel, "alg_bytes": B, "alg_flops": F,
+        "kernel_GBs": B / (ms_kernel * 1e-3) / 1e9, "kernel_TFLOPs": F / (ms_kernel * 1e-3) / 1e12,
+    }
+    res["hbm_frac"] = res["kernel_GBs"] / HBM_PEAK_GBS
+    del op, u, v
+    torch.cuda.empty_cache()
+    return res
+
+
+def cpu_baseline(p: int = 4, seconds: float = 10.0) -> dict | None:
+    """Reference /cpu/self/avx/blocked on this host's cores (one single-threaded process per core)."""
+    exe = ROOT / "oracle" / "_ref" / "ref_cpu_bench"
+    cores = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1))
+    n = 12  # 12^3 elements at p = 4: 117,649 DoFs per process
+    if exe.exists():
+        procs = [subprocess.Popen([str(exe), "/cpu/self/avx/blocked", str(p), str(n), str(seconds)], stdout=subprocess.PIPE, text=True)
+                 for _ in range(cores)]
+        rates, dofs = [], 0
+        for pr in procs:
+            out, _ = pr.communicate(timeout=seconds * 6 + 120)
+            if pr.returncode != 0:
+                return None
+            r = json.loads(out.strip().splitlines()[-1])
+            rates.append(r["dofs"] * r["applies"] / r["seconds"])
+            dofs = r["dofs"]
+        cpu_model = ""
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+        return {"value": sum(rates) / 1e9, "unit": "GDoF/s", "cores": cores, "kind": "reference",
+                "sample": f"/cpu/self/avx/blocked (reference sources, -O3 -march=x86-64-v3), 3D Poisson p={p} Q={p + 2}, "
+                          f"{n}^3 elements ({dofs} DoFs) per process, {cores} processes x {seconds:.0f} s",
+                "cpu": cpu_model}
+    # fallback: the oracle restatement of /cpu/self/ref/serial, one core
+    import oracle
+    from libceed_amd.mesh import hex_mesh
+
+    m = hex_mesh(8, p, seed=0)
+    qd = oracle.hex_build_qdata(1, m, p + 2)
+    u = np.random.default_rng(0).uniform(-1, 1, m.num_nodes)
+    k, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        oracle.hex_apply(1, 1, m.num_elem, p + 1, p + 2, m.offsets, m.num_nodes, qd, u)
+        k += 1
+    dt = time.perf_counter() - t0
+    return {"value": m.num_nodes * k / dt / 1e9, "unit": "GDoF/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (ref/serial restatement), p={p}, 8^3 elements, {dt:.0f} s"}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--p", type=int, default=4)
+    ap.add_argument("--dofs", type=float, default=8e6)
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it (see DESIGN.md, sharding)"}))
+        sys.exit(2)
+
+    import torch
+
+    from libceed_amd import ops
+
+    ops.device_init(0)
+    head = run_case(args.p, args.dofs, torch.float64, args.steps, args.warmup)
+    sweep = {}
+    if not args.no_sweep:
+        for p in range(1, 9):
+            r64 = head if p == args.p else run_case(p, args.dofs, torch.float64, max(10, args.steps // 4), 3)
+            r32 = run_case(p, args.dofs, torch.float32, max(10, args.steps // 4), 3)
+            sweep[str(p)] = {"dofs": r64["dofs"], "fp64_gdofs": round(r64["gdofs"], 3), "fp64_hbm_frac": round(r64["hbm_frac"], 4),
+                             "fp32_gdofs": round(r32["gdofs"], 3), "fp32_hbm_frac": round(r32["hbm_frac"], 4)}
+            print(f"# p={p}: {sweep[str(p)]}", file=sys.stderr, flush=True)
+    cpu = None if args.no_cpu else cpu_baseline(args.p)
+    line = {
+        "metric": "3D diffusion apply GDoF/s p=1..8 (FP64 vs FP32+refine @1e-10), % HBM peak",
+        "value": round(head["gdofs"], 4),
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_apply"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded perturbed structured hex mesh, u ~ U[-1,1]",
+        "config": {"workload": f"3D Poisson operator apply, FP64, p={args.p} (P={args.p + 1}, Q={args.p + 2}), "
+                               f"{head['n']}^3 hex elements, {head['dofs']} DoFs",
+                   "p": args.p, "dofs": head["dofs"], "elements": head["elements"], "element_colors": head["colors"]},
+        "roofline": {"bound": "hbm", "achieved": round(head["kernel_GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(head["hbm_frac"], 4), "traffic": None,
+                     "kernel": "hex_apply_kernel (all colour launches of one apply)",
+                     "alg_bytes_per_apply": head["alg_bytes"], "kernel_ms_per_apply": round(head["kernel_ms"], 5),
+                     "fp64_tflops": round(head["kernel_TFLOPs"], 2), "fp64_peak_tflops": FP64_PEAK_TFLOPS},
+        "cpu_baseline": cpu,
+        "sweep": sweep,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,144 @@
+/*
+ * ceed-mi355-kernels.h -- thin C ABI from host code to the hand-written HIP
+ * kernels for gfx950 (MI355X).  Plain pointers and sizes only.
+ *
+ * Every entry point is an MI355X-native replacement for one step of the
+ * reference hot path v = E^T B^T D B E u; the comment on each names the
+ * reference function (file:line in CEED/libCEED) whose result it reproduces.
+ *
+ * Conventions
+ *   - Return 0 on success, a negative CEED_MI355_ERROR_* code on failure; the
+ *     message is available from CeedMi355GetLastError() on the same thread.
+ *   - Pointers named *_host are host memory; every other array argument is
+ *     device memory (hipMalloc'ed or a torch CUDA tensor's data_ptr()).
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *   - Modes use the reference's enum values: t_mode 0 = NOTRANSPOSE,
+ *     1 = TRANSPOSE; eval_mode 1 = INTERP, 2 = GRAD, 16 = WEIGHT
+ *     (include/ceed/types.h:196-247).
+ *   - E-vectors are element-major [elem][comp][node]; basis Q-vectors are
+ *     [elem][dim][comp][qpt] (GRAD) or [elem][comp][qpt] (INTERP).
+ */
+#ifndef CEED_MI355_KERNELS_H
+#define CEED_MI355_KERNELS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes share the values of CeedErrorType (include/ceed/types.h:162-181). */
+enum {
+  CEED_MI355_SUCCESS           = 0,
+  CEED_MI355_ERROR_DIMENSION   = 2,
+  CEED_MI355_ERROR_MAJOR       = -1,
+  CEED_MI355_ERROR_BACKEND     = -2,
+  CEED_MI355_ERROR_UNSUPPORTED = -3,
+};
+
+/* Arithmetic type of an operator plan. */
+enum { CEED_MI355_FP64 = 0, CEED_MI355_FP32 = 1 };
+
+/* Pointwise operators the fused kernel inlines (the gallery QFunctions
+ * MassApply / Vector3MassApply, Poisson3DApply / Vector3Poisson3DApply, and a
+ * kappa-weighted Poisson3DApply for -div(kappa grad u)). */
+enum { CEED_MI355_QF_MASS = 0, CEED_MI355_QF_DIFFUSION = 1, CEED_MI355_QF_DIFFUSION_KAPPA = 2 };
+
+/* Geometric-factor builders (gallery Mass3DBuild / Poisson3DBuild). */
+enum { CEED_MI355_BUILD_MASS = 0, CEED_MI355_BUILD_DIFFUSION = 1 };
+
+/* ------------------------------------------------------------------------ */
+/* Device and memory                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *CeedMi355GetLastError(void);
+/* Select the device and check it is a gfx950; replaces the device set-up of
+ * backends/hip/ceed-hip-common.c:18-40 (CeedInit_Hip). */
+int CeedMi355DeviceInit(int device_id);
+int CeedMi355Malloc(size_t bytes, void **ptr);
+int CeedMi355Free(void *ptr);
+int CeedMi355Memcpy(void *dst, const void *src, size_t bytes, void *stream); /* any direction, stream-ordered */
+int CeedMi355Memset(void *dst, int value, size_t bytes, void *stream);
+int CeedMi355StreamSynchronize(void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Element restriction (E / E^T)                                             */
+/* ------------------------------------------------------------------------ */
+typedef struct CeedMi355Restriction_private *CeedMi355Restriction;
+
+/* Offset restriction.  Copies the offsets to the device and builds the
+ * transpose map used by the bit-exact E^T.  Replaces CeedElemRestrictionCreate
+ * on /cpu/self/ref/serial (backends/ref/ceed-ref-restriction.c:746-905). */
+int CeedMi355RestrictionCreateOffset(int32_t num_elem, int32_t elem_size, int32_t num_comp, int32_t comp_stride, int64_t l_size,
+                                     const int32_t *offsets_host, CeedMi355Restriction *rstr);
+/* Strided restriction: L index = node*strides[0] + comp*strides[1] + elem*strides[2]. */
+int CeedMi355RestrictionCreateStrided(int32_t num_elem, int32_t elem_size, int32_t num_comp, int64_t l_size, const int32_t *strides_host,
+                                      CeedMi355Restriction *rstr);
+/* NOTRANSPOSE: v(E-vector) = E u, replaces ceed-ref-restriction.c:19-75.
+ * TRANSPOSE:   v(L-vector) += E^T u, summed per L entry in ascending
+ *              (elem, comp, node) order, bit-identical to
+ *              ceed-ref-restriction.c:181-242 (serial, block size 1). */
+int CeedMi355RestrictionApply(CeedMi355Restriction rstr, int t_mode, const double *u, double *v, void *stream);
+int CeedMi355RestrictionDestroy(CeedMi355Restriction *rstr);
+
+/* ------------------------------------------------------------------------ */
+/* Tensor-product H1 basis (B / B^T), dim 1..3                                */
+/* ------------------------------------------------------------------------ */
+typedef struct CeedMi355Basis_private *CeedMi355Basis;
+
+/* interp_1d and grad_1d are Q x P row-major host arrays; q_weight_1d has Q
+ * entries.  Replaces CeedBasisCreateTensorH1_Ref (ceed-ref-basis.c:283-310). */
+int CeedMi355BasisCreateTensorH1(int32_t dim, int32_t num_comp, int32_t P_1d, int32_t Q_1d, const double *interp_1d_host,
+                                 const double *grad_1d_host, const double *q_weight_1d_host, CeedMi355Basis *basis);
+/* Replaces CeedBasisApply / CeedBasisApplyAdd (ceed-ref-basis.c:19-257;
+ * interface/ceed-basis.c:2083).  add != 0 sums into v. */
+int CeedMi355BasisApply(CeedMi355Basis basis, int32_t num_elem, int t_mode, int eval_mode, int add, const double *u, double *v,
+                        void *stream);
+int CeedMi355BasisDestroy(CeedMi355Basis *basis);
+
+/* ------------------------------------------------------------------------ */
+/* Fused hex operator  v (+)= E^T B^T D B E u                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct CeedMi355Operator_private *CeedMi355Operator;
+
+/* Plan for a 3D tensor hex operator with a gallery pointwise D.  Colours the
+ * elements so that no two elements of one colour share an L-vector entry,
+ * and stores offsets and geometric factors in colour order, so the fused
+ * kernel's E^T needs no atomics.  Supports Q_1d >= P_1d, P_1d in 2..10.
+ * scalar_type selects FP64 or FP32 storage and arithmetic for u, v and
+ * q_data.  Replaces the per-operator set-up of ceed-ref-operator.c:145-210. */
+int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_comp, int32_t num_elem, int32_t P_1d, int32_t Q_1d,
+                               const double *interp_1d_host, const double *grad_1d_host, const int32_t *offsets_host, int32_t comp_stride,
+                               int64_t l_size, CeedMi355Operator *op);
+/* Geometric factors from mesh coordinates on the GPU, FP64 arithmetic:
+ * Mass3DBuild (w det J) or Poisson3DBuild (6 Voigt entries of
+ * w det J J^-1 J^-T).  The mesh restriction has mesh_P^3 nodes per element,
+ * 3 components at mesh_comp_stride; coords is an FP64 L-vector.  Replaces
+ * applying the build operator of examples/ceed/ex1-volume.c:195-236 with the
+ * gallery QFunctions jit-source/gallery/ceed-{mass3d,poisson3d}build.h. */
+int CeedMi355OperatorBuildQData(CeedMi355Operator op, int build, int32_t mesh_P_1d, const double *mesh_interp_1d_host,
+                                const double *mesh_grad_1d_host, const double *q_weight_1d_host, const int32_t *mesh_offsets_host,
+                                int32_t mesh_comp_stride, const double *coords, void *stream);
+/* Load geometric factors from a caller array q_data (op scalar type) in the
+ * backend-strided layout [elem][comp][qpt] of CEED_STRIDES_BACKEND. */
+int CeedMi355OperatorSetQData(CeedMi355Operator op, const void *q_data, void *stream);
+/* Copy the geometric factors back out in [elem][comp][qpt] layout (FP64 or FP32 per the plan). */
+int CeedMi355OperatorGetQData(CeedMi355Operator op, void *q_data, void *stream);
+/* kappa at quadrature points, [elem][qpt] FP64 (QF_DIFFUSION_KAPPA only). */
+int CeedMi355OperatorSetKappa(CeedMi355Operator op, const double *kappa, void *stream);
+/* v += A u  (CeedOperatorApplyAdd, interface/ceed-operator.c:2313-2340). */
+int CeedMi355OperatorApplyAdd(CeedMi355Operator op, const void *u, void *v, void *stream);
+/* v  = A u  (CeedOperatorApply, interface/ceed-operator.c:2271-2294). */
+int CeedMi355OperatorApply(CeedMi355Operator op, const void *u, void *v, void *stream);
+/* Number of element colours, and the element count of each (host array of at
+ * least num_colors entries, may be NULL). */
+int CeedMi355OperatorGetColors(CeedMi355Operator op, int32_t *num_colors, int32_t *color_sizes_host);
+int CeedMi355OperatorDestroy(CeedMi355Operator *op);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CEED_MI355_KERNELS_H */

@@ -1,0 +1,35 @@
+/*
+ * ceed-mi355.h -- host-side entry points of the /gpu/hip/mi355 backend.
+ *
+ * The fused kernels are reached through the thin C ABI in
+ * ceed-mi355-kernels.h; this header holds the host helpers the backend and
+ * its users need (quadrature, 1D Lagrange matrices).
+ */
+#ifndef CEED_MI355_H
+#define CEED_MI355_H
+
+#include <ceed/types.h>
+
+#include "ceed-mi355-kernels.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Gauss-Legendre points/weights on [-1, 1]; same signature and result as the
+ * reference's public CeedGaussQuadrature (interface/ceed-basis.c:2529). */
+int CeedGaussQuadrature(CeedInt Q, CeedScalar *q_ref_1d, CeedScalar *q_weight_1d);
+/* Gauss-Lobatto points/weights; q_weight_1d may be NULL.  Replaces
+ * CeedLobattoQuadrature (interface/ceed-basis.c:2581). */
+int CeedLobattoQuadrature(CeedInt Q, CeedScalar *q_ref_1d, CeedScalar *q_weight_1d);
+/* 1D Lagrange matrices on P Gauss-Lobatto nodes evaluated at Q points
+ * (quad_mode 0 = Gauss, 1 = Gauss-Lobatto): interp_1d, grad_1d are Q x P row
+ * major.  The matrices CeedBasisCreateTensorH1Lagrange builds
+ * (interface/ceed-basis.c:1617-1690). */
+int CeedMi355LagrangeBasis1D(int32_t P, int32_t Q, int32_t quad_mode, double *interp_1d, double *grad_1d, double *q_ref_1d, double *q_weight_1d);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CEED_MI355_H */

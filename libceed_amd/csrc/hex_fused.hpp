@@ -1,0 +1,231 @@
+// Fused per-element hex operator kernels for gfx950:
+//   gather (E) -> sum-factorised interp / collocated grad (B) -> pointwise D
+//   -> B^T -> non-atomic scatter-add (E^T, element colours never share a node).
+//
+// Thread layout: each element owns a T1 x T1 slab of threads (T1 = Q >= P),
+// thread (tx, ty) keeps the z-column of its (x, y) point in registers.  The
+// x and y contractions go through LDS slabs (one T1^3 tile per slab), the z
+// contraction stays in registers.  Several elements share a workgroup; the 1D
+// matrices B (Q x P) and D = G pinv(B) (Q x Q) are staged once per workgroup.
+//
+// Data layout in HBM (colour order: elements of colour c are contiguous):
+//   offsets [elem][P^3]   int32
+//   q_data  [elem][QC][Q^3]   QC = 1 (mass), 6 (diffusion), 7 (6 + kappa)
+//   u, v    L-vectors, component k of node n at n + k*comp_stride
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mi355 {
+
+enum { kQfMass = 0, kQfDiffusion = 1, kQfDiffusionKappa = 2 };
+
+template <int QF>
+struct QfTraits;
+template <>
+struct QfTraits<kQfMass> {
+  static constexpr int kComps = 1, kSlabs = 2;
+};
+template <>
+struct QfTraits<kQfDiffusion> {
+  static constexpr int kComps = 6, kSlabs = 3;
+};
+template <>
+struct QfTraits<kQfDiffusionKappa> {
+  static constexpr int kComps = 7, kSlabs = 3;
+};
+
+template <int P, int Q>
+struct HexShape {
+  static_assert(Q >= P, "fused hex kernel needs Q >= P (collocated gradient)");
+  static constexpr int T1 = Q, NT = Q * Q, SLAB = Q * Q * Q, P3 = P * P * P, Q3 = Q * Q * Q;
+};
+
+// LDS entries one workgroup needs: matrices + per-element slabs.
+template <int P, int Q, int QF>
+__host__ __device__ constexpr int hex_lds_entries(int epb) {
+  return Q * P + Q * Q + epb * QfTraits<QF>::kSlabs * Q * Q * Q;
+}
+
+// Pointwise Poisson3DApply (jit-source/gallery/ceed-poisson3dapply.h):
+// vg_j = sum_i ug_i S_ij with S in Voigt order {xx, yy, zz, yz, xz, xy}.
+template <typename T>
+__device__ __forceinline__ void poisson_point(const T *qd, int64_t stride, T dx, T dy, T dz, T &vx, T &vy, T &vz) {
+  const T s0 = qd[0], s1 = qd[stride], s2 = qd[2 * stride], s3 = qd[3 * stride], s4 = qd[4 * stride], s5 = qd[5 * stride];
+  vx = dx * s0 + dy * s5 + dz * s4;
+  vy = dx * s5 + dy * s1 + dz * s3;
+  vz = dx * s4 + dy * s3 + dz * s2;
+}
+
+template <int P, int Q, int NCOMP, int QF, typename T>
+__global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int64_t elem_end, const int32_t *__restrict__ offsets,
+                                                         int32_t comp_stride, const T *__restrict__ q_data, const T *__restrict__ mats,
+                                                         const T *__restrict__ u, T *__restrict__ v) {
+  using S = HexShape<P, Q>;
+  constexpr int T1 = S::T1, NT = S::NT, SLAB = S::SLAB, P3 = S::P3, Q3 = S::Q3, QC = QfTraits<QF>::kComps;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *sB = reinterpret_cast<T *>(lds_raw);
+  T *sD = sB + Q * P;
+
+  const int epb = blockDim.x / NT;
+  const int le = threadIdx.x / NT, t = threadIdx.x - le * NT;
+  const int tx = t % T1, ty = t / T1;
+  T        *S0 = sD + Q * Q + (size_t)le * QfTraits<QF>::kSlabs * SLAB;
+  T        *S1 = S0 + SLAB;
+  T        *S2 = S1 + SLAB;  // only used by the diffusion variants
+  for (int i = threadIdx.x; i < Q * P + Q * Q; i += blockDim.x) sB[i] = mats[i];
+
+  const int64_t e    = elem_begin + (int64_t)blockIdx.x * epb + le;
+  const bool    live = le < epb && e < elem_end;
+  const bool    node = live && tx < P && ty < P;
+  auto sidx = [](int z, int y, int x) { return (z * T1 + y) * T1 + x; };
+
+  int off[P];
+  if (node) {
+#pragma unroll
+    for (int z = 0; z < P; z++) off[z] = offsets[e * P3 + (z * P + ty) * P + tx];
+  }
+  const T *qd = q_data + (live ? e : 0) * (int64_t)QC * Q3 + (ty * Q + tx);
+  __syncthreads();
+
+  for (int comp = 0; comp < NCOMP; comp++) {
+    const int64_t coff = (int64_t)comp * comp_stride;
+    // ---- E: gather the element's nodes of this component into slab S0[z][y][x]
+    if (node) {
+#pragma unroll
+      for (int z = 0; z < P; z++) S0[sidx(z, ty, tx)] = u[off[z] + coff];
+    }
+    __syncthreads();
+    // ---- B, x direction: S1[z][y][qx] = sum_b B[qx][b] S0[z][y][b]   (y, z < P)
+    if (ty < P) {
+      T acc[P];
+#pragma unroll
+      for (int z = 0; z < P; z++) acc[z] = 0;
+#pragma unroll
+      for (int b = 0; b < P; b++) {
+        const T bb = sB[tx * P + b];
+#pragma unroll
+        for (int z = 0; z < P; z++) acc[z] += bb * S0[sidx(z, ty, b)];
+      }
+#pragma unroll
+      for (int z = 0; z < P; z++) S1[sidx(z, ty, tx)] = acc[z];
+    }
+    __syncthreads();
+    // ---- B, y then z direction (z in registers): uq[qz] at point (tx, ty, qz)
+    T uq[Q];
+    {
+      T acc[P];
+#pragma unroll
+      for (int z = 0; z < P; z++) acc[z] = 0;
+#pragma unroll
+      for (int b = 0; b < P; b++) {
+        const T bb = sB[ty * P + b];
+#pragma unroll
+        for (int z = 0; z < P; z++) acc[z] += bb * S1[sidx(z, b, tx)];
+      }
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) {
+        T s = 0;
+#pragma unroll
+        for (int b = 0; b < P; b++) s += sB[qz * P + b] * acc[b];
+        uq[qz] = s;
+      }
+    }
+    T t2[P];  // z-direction B^T result, per (tx, ty) column
+    if constexpr (QF == kQfMass) {
+      // ---- D: v = u * (w det J)
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) uq[qz] *= live ? qd[qz * Q * Q] : T(0);
+#pragma unroll
+      for (int b = 0; b < P; b++) {
+        T s = 0;
+#pragma unroll
+        for (int qz = 0; qz < Q; qz++) s += sB[qz * P + b] * uq[qz];
+        t2[b] = s;
+      }
+    } else {
+      // ---- collocated gradient: x, y through LDS, z in registers
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) S0[sidx(qz, ty, tx)] = uq[qz];
+      __syncthreads();
+      T vz[Q];
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) {
+        T dx = 0, dy = 0, dz = 0;
+#pragma unroll
+        for (int c = 0; c < Q; c++) {
+          dx += sD[tx * Q + c] * S0[sidx(qz, ty, c)];
+          dy += sD[ty * Q + c] * S0[sidx(qz, c, tx)];
+          dz += sD[qz * Q + c] * uq[c];
+        }
+        T vx = 0, vy = 0, vzz = 0;
+        if (live) {
+          const T *q = qd + qz * Q * Q;
+          poisson_point<T>(q, Q3, dx, dy, dz, vx, vy, vzz);
+          if constexpr (QF == kQfDiffusionKappa) {
+            const T k = q[6 * Q3];
+            vx *= k;
+            vy *= k;
+            vzz *= k;
+          }
+        }
+        S1[sidx(qz, ty, tx)] = vx;
+        S2[sidx(qz, ty, tx)] = vy;
+        vz[qz]               = vzz;
+      }
+      __syncthreads();
+      // ---- G^T: sum of the three direction transposes, then B^T in z
+      T acc[Q];
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) {
+        T s = 0;
+#pragma unroll
+        for (int a = 0; a < Q; a++) {
+          s += sD[a * Q + tx] * S1[sidx(qz, ty, a)];
+          s += sD[a * Q + ty] * S2[sidx(qz, a, tx)];
+          s += sD[a * Q + qz] * vz[a];
+        }
+        acc[qz] = s;
+      }
+#pragma unroll
+      for (int b = 0; b < P; b++) {
+        T s = 0;
+#pragma unroll
+        for (int qz = 0; qz < Q; qz++) s += sB[qz * P + b] * acc[qz];
+        t2[b] = s;
+      }
+    }
+    // ---- B^T, y direction: S1[z][y][qx] = sum_qy B[qy][y] S0[z][qy][qx]   (y, z < P)
+#pragma unroll
+    for (int b = 0; b < P; b++) S0[sidx(b, ty, tx)] = t2[b];
+    __syncthreads();
+    if (ty < P) {
+      T acc[P];
+#pragma unroll
+      for (int z = 0; z < P; z++) acc[z] = 0;
+#pragma unroll
+      for (int a = 0; a < Q; a++) {
+        const T bb = sB[a * P + ty];
+#pragma unroll
+        for (int z = 0; z < P; z++) acc[z] += bb * S0[sidx(z, a, tx)];
+      }
+#pragma unroll
+      for (int z = 0; z < P; z++) S1[sidx(z, ty, tx)] = acc[z];
+    }
+    __syncthreads();
+    // ---- B^T, x direction, then E^T: scatter-add (no atomics: colour-exclusive)
+    if (node) {
+#pragma unroll
+      for (int z = 0; z < P; z++) {
+        T s = 0;
+#pragma unroll
+        for (int a = 0; a < Q; a++) s += sB[a * P + tx] * S1[sidx(z, ty, a)];
+        T *dst = &v[off[z] + coff];
+        *dst += s;
+      }
+    }
+  }
+}
+
+}  // namespace mi355

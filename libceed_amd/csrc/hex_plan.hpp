@@ -1,0 +1,84 @@
+// Operator plan shared by the plan code (operator_hex.hip) and the kernel
+// instantiation units (hex_apply_f64.hip, hex_apply_f32.hip).
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "hex_fused.hpp"
+#include "mi355_common.h"
+
+struct CeedMi355Operator_private;
+
+namespace mi355 {
+
+using ApplyFn = int (*)(const CeedMi355Operator_private *op, const void *u, void *v, hipStream_t s);
+
+// Kernel launcher for one (P, Q, num_comp, QF, scalar type); nullptr when no instantiation exists.
+ApplyFn pick_apply_f64(int P, int Q, int num_comp, int qf);
+ApplyFn pick_apply_f32(int P, int Q, int num_comp, int qf);
+// Elements per workgroup giving ~256 threads and at most 64 KiB of LDS.
+int default_epb(int P, int Q, int qf, int scalar_bytes);
+int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes);
+
+}  // namespace mi355
+
+struct CeedMi355Operator_private {
+  int     qf, scalar_type, scalar_bytes, num_comp, P, Q, q_comp;
+  int32_t num_elem, comp_stride;
+  int64_t l_size;
+  int     epb;
+  // colour c owns packed elements [color_start[c], color_start[c+1])
+  std::vector<int64_t> color_start;
+  std::vector<int32_t> perm;  // packed position -> caller's element index
+  int32_t             *d_offsets = nullptr;  // [elem][P^3], packed order
+  int32_t             *d_perm    = nullptr;
+  void                *d_mats    = nullptr;  // B (Q x P) then D (Q x Q), plan scalar type
+  void                *d_qdata   = nullptr;  // [elem][q_comp][Q^3], packed order
+  mi355::ApplyFn       apply     = nullptr;
+};
+
+namespace mi355 {
+
+template <int P, int Q, int NC, int QF, typename T>
+int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v, hipStream_t s) {
+  const int    nt  = Q * Q;
+  const int    epb = op->epb;
+  const size_t lds = sizeof(T) * (size_t)hex_lds_entries<P, Q, QF>(epb);
+  for (size_t c = 0; c + 1 < op->color_start.size(); c++) {
+    const int64_t b = op->color_start[c], e = op->color_start[c + 1];
+    if (e <= b) continue;
+    const int64_t grid = (e - b + epb - 1) / epb;
+    hex_apply_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, op->d_offsets, op->comp_stride, (const T *)op->d_qdata,
+                                                                          (const T *)op->d_mats, (const T *)u, (T *)v);
+  }
+  MI355_LAUNCH_CHECK();
+  return CEED_MI355_SUCCESS;
+}
+
+// Dispatch over the instantiated (P, Q) pairs: P = 2..9, Q in {P, P+1}.
+template <typename T>
+ApplyFn pick_apply(int P, int Q, int nc, int qf) {
+#define MI355_CASE_PQ(PP, QQ)                                                                     \
+  if (P == PP && Q == QQ) {                                                                       \
+    if (qf == kQfMass) return nc == 1 ? launch_hex_apply<PP, QQ, 1, kQfMass, T>                   \
+                                      : (nc == 3 ? launch_hex_apply<PP, QQ, 3, kQfMass, T> : nullptr); \
+    if (qf == kQfDiffusion) return nc == 1 ? launch_hex_apply<PP, QQ, 1, kQfDiffusion, T>         \
+                                           : (nc == 3 ? launch_hex_apply<PP, QQ, 3, kQfDiffusion, T> : nullptr); \
+    if (qf == kQfDiffusionKappa) return nc == 1 ? launch_hex_apply<PP, QQ, 1, kQfDiffusionKappa, T> : nullptr; \
+    return nullptr;                                                                               \
+  }
+  MI355_CASE_PQ(2, 2) MI355_CASE_PQ(2, 3)
+  MI355_CASE_PQ(3, 3) MI355_CASE_PQ(3, 4)
+  MI355_CASE_PQ(4, 4) MI355_CASE_PQ(4, 5)
+  MI355_CASE_PQ(5, 5) MI355_CASE_PQ(5, 6)
+  MI355_CASE_PQ(6, 6) MI355_CASE_PQ(6, 7)
+  MI355_CASE_PQ(7, 7) MI355_CASE_PQ(7, 8)
+  MI355_CASE_PQ(8, 8) MI355_CASE_PQ(8, 9)
+  MI355_CASE_PQ(9, 9) MI355_CASE_PQ(9, 10)
+#undef MI355_CASE_PQ
+  return nullptr;
+}
+
+}  // namespace mi355

@@ -1,0 +1,415 @@
+// Fused hex operator plans: element colouring, colour-ordered data layout,
+// geometric factors on the GPU, and the apply entry points.
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "hex_plan.hpp"
+
+namespace mi355 {
+void collocated_grad(int P, int Q, const double *interp, const double *grad, double *collo);
+
+int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes) {
+  const int slabs = qf == kQfMass ? 2 : 3;
+  return scalar_bytes * (Q * P + Q * Q + epb * slabs * Q * Q * Q);
+}
+
+int default_epb(int P, int Q, int qf, int scalar_bytes) {
+  int epb = 256 / (Q * Q);
+  if (epb < 1) epb = 1;
+  while (epb > 1 && lds_bytes(P, Q, qf, epb, scalar_bytes) > 64 * 1024) epb--;
+  return epb;
+}
+}  // namespace mi355
+
+namespace {
+
+using mi355::kQfDiffusion;
+using mi355::kQfDiffusionKappa;
+using mi355::kQfMass;
+
+template <typename T>
+__global__ void __launch_bounds__(256) fill_kernel(int64_t n, T value, T *__restrict__ x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = value;
+}
+
+// Set entries [slot, slot + per) of every element block (block stride `stride`) to `value`.
+template <typename T>
+__global__ void __launch_bounds__(256) fill_slot(int64_t num_elem, int64_t per, int64_t stride, int64_t slot, T value, T *__restrict__ x) {
+  const int64_t n = num_elem * per;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / per;
+    x[p * stride + slot + (i - p * per)] = value;
+  }
+}
+
+// Copy per-element blocks of `per` entries between the caller's element order
+// and the plan's colour order.  to_packed: dst[p] = src[perm[p]].
+template <typename TS, typename TD>
+__global__ void __launch_bounds__(256) permute_elements(int64_t num_elem, int64_t per, const int32_t *__restrict__ perm, int to_packed,
+                                                        const TS *__restrict__ src, TD *__restrict__ dst) {
+  const int64_t n = num_elem * per;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / per, r = i - p * per, o = perm[p];
+    if (to_packed) dst[i] = (TD)src[o * per + r];
+    else dst[o * per + r] = (TD)src[i];
+  }
+}
+
+// Geometric factors from coordinates, FP64 arithmetic.  One Q x Q thread slab
+// per element; mesh basis has PM <= Q nodes per direction (runtime).
+// Mass3DBuild:    q = w det J                    (gallery/ceed-mass3dbuild.h)
+// Poisson3DBuild: q = Voigt of (w/det J) adj(J) adj(J)^T (gallery/ceed-poisson3dbuild.h)
+template <int Q, int BUILD, typename TOUT>
+__global__ void __launch_bounds__(1024) hex_build_kernel(int64_t num_elem, int PM, const int32_t *__restrict__ mesh_offsets,
+                                                         int32_t mesh_comp_stride, const double *__restrict__ coords,
+                                                         const double *__restrict__ mats, int64_t out_elem_stride,
+                                                         TOUT *__restrict__ q_data) {
+  constexpr int T1 = Q, NT = Q * Q, SLAB = Q * Q * Q, QC = BUILD == 0 ? 1 : 6;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double *sB = reinterpret_cast<double *>(lds_raw);  // Q x PM
+  double *sD = sB + Q * PM;                          // Q x Q
+  double *sW = sD + Q * Q;                           // Q
+  const int epb = blockDim.x / NT, le = threadIdx.x / NT, t = threadIdx.x - le * NT, tx = t % T1, ty = t / T1;
+  double   *S0 = sW + Q + (size_t)le * 5 * SLAB, *S1 = S0 + SLAB, *X = S1 + SLAB;  // X: 3 slabs of coordinates at qpts
+  for (int i = threadIdx.x; i < Q * PM + Q * Q + Q; i += blockDim.x) sB[i] = mats[i];
+  const int64_t e    = (int64_t)blockIdx.x * epb + le;
+  const bool    live = e < num_elem;
+  const int     PM3  = PM * PM * PM;
+  auto sidx = [](int z, int y, int x) { return (z * T1 + y) * T1 + x; };
+  __syncthreads();
+  for (int c = 0; c < 3; c++) {
+    if (live && tx < PM && ty < PM)
+      for (int z = 0; z < PM; z++) S0[sidx(z, ty, tx)] = coords[mesh_offsets[e * PM3 + (z * PM + ty) * PM + tx] + (int64_t)c * mesh_comp_stride];
+    __syncthreads();
+    if (ty < PM) {
+      for (int z = 0; z < PM; z++) {
+        double s = 0;
+        for (int b = 0; b < PM; b++) s += sB[tx * PM + b] * S0[sidx(z, ty, b)];
+        S1[sidx(z, ty, tx)] = s;
+      }
+    }
+    __syncthreads();
+    double acc[Q];
+    for (int z = 0; z < PM; z++) {
+      double s = 0;
+      for (int b = 0; b < PM; b++) s += sB[ty * PM + b] * S1[sidx(z, b, tx)];
+      acc[z] = s;
+    }
+    double *Xc = X + c * SLAB;
+    for (int qz = 0; qz < Q; qz++) {
+      double s = 0;
+      for (int b = 0; b < PM; b++) s += sB[qz * PM + b] * acc[b];
+      Xc[sidx(qz, ty, tx)] = s;
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  const int64_t Q3 = SLAB;
+  for (int qz = 0; qz < Q; qz++) {
+    double J[3][3];  // J[d][c] = d x_c / d X_d
+    for (int c = 0; c < 3; c++) {
+      const double *Xc = X + c * SLAB;
+      double        d0 = 0, d1 = 0, d2 = 0;
+      for (int k = 0; k < Q; k++) {
+        d0 += sD[tx * Q + k] * Xc[sidx(qz, ty, k)];
+        d1 += sD[ty * Q + k] * Xc[sidx(qz, k, tx)];
+        d2 += sD[qz * Q + k] * Xc[sidx(k, ty, tx)];
+      }
+      J[0][c] = d0;
+      J[1][c] = d1;
+      J[2][c] = d2;
+    }
+    const double  w   = sW[qz] * (sW[ty] * sW[tx]);
+    const int64_t pt  = (qz * Q + ty) * Q + tx;
+    TOUT         *out = q_data + e * out_elem_stride + pt;
+    if (BUILD == 0) {
+      const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      out[0] = (TOUT)(det * w);
+    } else {
+      double A[3][3];
+      for (int j = 0; j < 3; j++)
+        for (int k = 0; k < 3; k++)
+          A[k][j] = J[(k + 1) % 3][(j + 1) % 3] * J[(k + 2) % 3][(j + 2) % 3] - J[(k + 2) % 3][(j + 1) % 3] * J[(k + 1) % 3][(j + 2) % 3];
+      const double qw = w / (J[0][0] * A[0][0] + J[0][1] * A[0][1] + J[0][2] * A[0][2]);
+      out[0 * Q3] = (TOUT)(qw * (A[0][0] * A[0][0] + A[0][1] * A[0][1] + A[0][2] * A[0][2]));
+      out[1 * Q3] = (TOUT)(qw * (A[1][0] * A[1][0] + A[1][1] * A[1][1] + A[1][2] * A[1][2]));
+      out[2 * Q3] = (TOUT)(qw * (A[2][0] * A[2][0] + A[2][1] * A[2][1] + A[2][2] * A[2][2]));
+      out[3 * Q3] = (TOUT)(qw * (A[1][0] * A[2][0] + A[1][1] * A[2][1] + A[1][2] * A[2][2]));
+      out[4 * Q3] = (TOUT)(qw * (A[0][0] * A[2][0] + A[0][1] * A[2][1] + A[0][2] * A[2][2]));
+      out[5 * Q3] = (TOUT)(qw * (A[0][0] * A[1][0] + A[0][1] * A[1][1] + A[0][2] * A[1][2]));
+    }
+  }
+}
+
+template <int Q, typename TOUT>
+int launch_build(int build, int64_t num_elem, int PM, const int32_t *mesh_offsets, int32_t mcs, const double *coords, const double *mats,
+                 int64_t stride, TOUT *q_data, hipStream_t s) {
+  const int nt = Q * Q;
+  int       epb = std::max(1, 256 / nt);
+  while (epb > 1 && sizeof(double) * (Q * PM + Q * Q + Q + (size_t)epb * 5 * Q * Q * Q) > 64 * 1024) epb--;
+  const size_t lds = sizeof(double) * (Q * PM + Q * Q + Q + (size_t)epb * 5 * Q * Q * Q);
+  const int64_t grid = (num_elem + epb - 1) / epb;
+  if (build == 0)
+    hex_build_kernel<Q, 0, TOUT><<<(unsigned)grid, epb * nt, lds, s>>>(num_elem, PM, mesh_offsets, mcs, coords, mats, stride, q_data);
+  else hex_build_kernel<Q, 1, TOUT><<<(unsigned)grid, epb * nt, lds, s>>>(num_elem, PM, mesh_offsets, mcs, coords, mats, stride, q_data);
+  MI355_LAUNCH_CHECK();
+  return CEED_MI355_SUCCESS;
+}
+
+template <typename TOUT>
+int dispatch_build(int Q, int build, int64_t num_elem, int PM, const int32_t *mo, int32_t mcs, const double *coords, const double *mats,
+                   int64_t st, TOUT *q_data, hipStream_t s) {
+  switch (Q) {
+    case 2: return launch_build<2, TOUT>(build, num_elem, PM, mo, mcs, coords, mats, st, q_data, s);
+    case 3: return launch_build<3, TOUT>(build, num_elem, PM, mo, mcs, coords, mats, st, q_data, s);
+    case 4: return launch_build<4, TOUT>(build, num_elem, PM, mo, mcs, coords, mats, st, q_data, s);
+    case 5: return launch_build<5, TOUT>(build, num_elem, PM, mo, mcs, coords, mats, st, q_data, s);
+    case 6: return launch_build<6, TOUT>(build, num_elem, PM, mo, mcs, coords, mats, st, q_data, s);
+    case 7: return launch_build<7, TOUT>(build, num_elem, PM, mo, mcs, coords, mats, st, q_data, s);
+    case 8: return launch_build<8, TOUT>(build, num_elem, PM, mo, mcs, coords, mats, st, q_data, s);
+    case 9: return launch_build<9, TOUT>(build, num_elem, PM, mo, mcs, coords, mats, st, q_data, s);
+    case 10: return launch_build<10, TOUT>(build, num_elem, PM, mo, mcs, coords, mats, st, q_data, s);
+  }
+  MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED, "geometric factors: Q=%d outside 2..10", Q);
+}
+
+// Greedy colouring in ascending element order over every L index an element
+// touches: an element takes the smallest colour none of its entries carries.
+int color_elements(const int32_t *offsets, int64_t num_elem, int es, int nc, int64_t cs, int64_t l_size, std::vector<int32_t> &color,
+                   int &num_colors) {
+  std::vector<uint64_t> mask((size_t)l_size, 0);
+  color.assign((size_t)num_elem, 0);
+  num_colors = 0;
+  for (int64_t e = 0; e < num_elem; e++) {
+    uint64_t used = 0;
+    for (int i = 0; i < es; i++)
+      for (int k = 0; k < nc; k++) {
+        const int64_t l = (int64_t)offsets[e * es + i] + k * cs;
+        MI355_CHECK(l >= 0 && l < l_size, CEED_MI355_ERROR_DIMENSION, "offset %lld outside L-vector of size %lld", (long long)l,
+                    (long long)l_size);
+        used |= mask[l];
+      }
+    MI355_CHECK(~used != 0, CEED_MI355_ERROR_UNSUPPORTED, "element colouring needs more than 64 colours");
+    int c = 0;
+    while (used & (1ull << c)) c++;
+    color[e] = c;
+    num_colors = std::max(num_colors, c + 1);
+    for (int i = 0; i < es; i++)
+      for (int k = 0; k < nc; k++) mask[(int64_t)offsets[e * es + i] + k * cs] |= 1ull << c;
+  }
+  return CEED_MI355_SUCCESS;
+}
+
+int64_t elem_block(const CeedMi355Operator op) {
+  return (int64_t)op->q_comp * op->Q * op->Q * op->Q;
+}
+
+}  // namespace
+
+extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_comp, int32_t num_elem, int32_t P_1d, int32_t Q_1d,
+                                          const double *interp_1d_host, const double *grad_1d_host, const int32_t *offsets_host,
+                                          int32_t comp_stride, int64_t l_size, CeedMi355Operator *op_out) {
+  *op_out = nullptr;
+  MI355_CHECK(qf == kQfMass || qf == kQfDiffusion || qf == kQfDiffusionKappa, CEED_MI355_ERROR_UNSUPPORTED, "unknown pointwise operator %d", qf);
+  MI355_CHECK(scalar_type == CEED_MI355_FP64 || scalar_type == CEED_MI355_FP32, CEED_MI355_ERROR_UNSUPPORTED, "unknown scalar type");
+  MI355_CHECK(num_elem >= 0 && l_size > 0, CEED_MI355_ERROR_DIMENSION, "invalid operator sizes");
+  mi355::ApplyFn fn = scalar_type == CEED_MI355_FP64 ? mi355::pick_apply_f64(P_1d, Q_1d, num_comp, qf)
+                                                     : mi355::pick_apply_f32(P_1d, Q_1d, num_comp, qf);
+  MI355_CHECK(fn, CEED_MI355_ERROR_UNSUPPORTED, "no fused kernel for P=%d Q=%d num_comp=%d qf=%d", P_1d, Q_1d, num_comp, qf);
+
+  CeedMi355Operator op = new CeedMi355Operator_private();
+  op->qf = qf;
+  op->scalar_type = scalar_type;
+  op->scalar_bytes = scalar_type == CEED_MI355_FP64 ? 8 : 4;
+  op->num_comp = num_comp;
+  op->P = P_1d;
+  op->Q = Q_1d;
+  op->q_comp = qf == kQfMass ? 1 : (qf == kQfDiffusion ? 6 : 7);
+  op->num_elem = num_elem;
+  op->comp_stride = comp_stride;
+  op->l_size = l_size;
+  op->apply = fn;
+  op->epb = mi355::default_epb(P_1d, Q_1d, qf, op->scalar_bytes);
+  if (const char *env = getenv("CEED_MI355_EPB")) {
+    int v = atoi(env);
+    if (v >= 1 && mi355::lds_bytes(P_1d, Q_1d, qf, v, op->scalar_bytes) <= 64 * 1024 && v * Q_1d * Q_1d <= 256) op->epb = v;
+  }
+
+  const int            es = P_1d * P_1d * P_1d;
+  std::vector<int32_t> color;
+  int                  num_colors = 0;
+  int ierr = color_elements(offsets_host, num_elem, es, num_comp, comp_stride, l_size, color, num_colors);
+  if (ierr) {
+    delete op;
+    return ierr;
+  }
+  // packed order: by colour, ascending element index inside a colour
+  op->color_start.assign(num_colors + 1, 0);
+  for (int64_t e = 0; e < num_elem; e++) op->color_start[color[e] + 1]++;
+  for (int c = 0; c < num_colors; c++) op->color_start[c + 1] += op->color_start[c];
+  op->perm.resize(num_elem);
+  {
+    std::vector<int64_t> cursor(op->color_start.begin(), op->color_start.end() - 1);
+    for (int64_t e = 0; e < num_elem; e++) op->perm[cursor[color[e]]++] = (int32_t)e;
+  }
+  std::vector<int32_t> packed((size_t)num_elem * es);
+  for (int64_t p = 0; p < num_elem; p++) memcpy(&packed[p * es], &offsets_host[(int64_t)op->perm[p] * es], sizeof(int32_t) * es);
+
+  // B and the collocated derivative D = G pinv(B), in the plan's scalar type
+  std::vector<double> mats(Q_1d * P_1d + Q_1d * Q_1d);
+  memcpy(mats.data(), interp_1d_host, sizeof(double) * Q_1d * P_1d);
+  mi355::collocated_grad(P_1d, Q_1d, interp_1d_host, grad_1d_host, &mats[Q_1d * P_1d]);
+
+  MI355_HIP(hipMalloc(&op->d_offsets, sizeof(int32_t) * (packed.size() + 1)));
+  MI355_HIP(hipMemcpy(op->d_offsets, packed.data(), sizeof(int32_t) * packed.size(), hipMemcpyHostToDevice));
+  MI355_HIP(hipMalloc(&op->d_perm, sizeof(int32_t) * (num_elem + 1)));
+  MI355_HIP(hipMemcpy(op->d_perm, op->perm.data(), sizeof(int32_t) * num_elem, hipMemcpyHostToDevice));
+  MI355_HIP(hipMalloc(&op->d_mats, op->scalar_bytes * mats.size()));
+  if (scalar_type == CEED_MI355_FP64) {
+    MI355_HIP(hipMemcpy(op->d_mats, mats.data(), sizeof(double) * mats.size(), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> m32(mats.begin(), mats.end());
+    MI355_HIP(hipMemcpy(op->d_mats, m32.data(), sizeof(float) * m32.size(), hipMemcpyHostToDevice));
+  }
+  MI355_HIP(hipMalloc(&op->d_qdata, (size_t)op->scalar_bytes * ((size_t)num_elem * elem_block(op) + 1)));
+  MI355_HIP(hipMemset(op->d_qdata, 0, (size_t)op->scalar_bytes * ((size_t)num_elem * elem_block(op) + 1)));
+  *op_out = op;
+  return CEED_MI355_SUCCESS;
+}
+
+extern "C" int CeedMi355OperatorBuildQData(CeedMi355Operator op, int build, int32_t mesh_P_1d, const double *mesh_interp_1d_host,
+                                           const double *mesh_grad_1d_host, const double *q_weight_1d_host, const int32_t *mesh_offsets_host,
+                                           int32_t mesh_comp_stride, const double *coords, void *stream) {
+  MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
+  MI355_CHECK((build == 0 && op->qf == kQfMass) || (build == 1 && op->qf != kQfMass), CEED_MI355_ERROR_BACKEND,
+              "geometric factor kind %d does not match the operator", build);
+  MI355_CHECK(mesh_P_1d >= 2 && mesh_P_1d <= op->Q, CEED_MI355_ERROR_UNSUPPORTED, "mesh P_1d=%d must be in 2..Q=%d", mesh_P_1d, op->Q);
+  hipStream_t s = (hipStream_t)stream;
+  const int   Q = op->Q, PM = mesh_P_1d, PM3 = PM * PM * PM;
+  // mesh offsets in the plan's colour order
+  std::vector<int32_t> mo((size_t)op->num_elem * PM3);
+  for (int64_t p = 0; p < op->num_elem; p++) memcpy(&mo[p * PM3], &mesh_offsets_host[(int64_t)op->perm[p] * PM3], sizeof(int32_t) * PM3);
+  std::vector<double> mats(Q * PM + Q * Q + Q);
+  memcpy(mats.data(), mesh_interp_1d_host, sizeof(double) * Q * PM);
+  mi355::collocated_grad(PM, Q, mesh_interp_1d_host, mesh_grad_1d_host, &mats[Q * PM]);
+  memcpy(&mats[Q * PM + Q * Q], q_weight_1d_host, sizeof(double) * Q);
+  int32_t *d_mo   = nullptr;
+  double  *d_mats = nullptr;
+  MI355_HIP(hipMalloc(&d_mo, sizeof(int32_t) * (mo.size() + 1)));
+  MI355_HIP(hipMalloc(&d_mats, sizeof(double) * mats.size()));
+  MI355_HIP(hipMemcpyAsync(d_mo, mo.data(), sizeof(int32_t) * mo.size(), hipMemcpyHostToDevice, s));
+  MI355_HIP(hipMemcpyAsync(d_mats, mats.data(), sizeof(double) * mats.size(), hipMemcpyHostToDevice, s));
+  int ierr;
+  const int64_t blk = elem_block(op);
+  if (op->scalar_type == CEED_MI355_FP64)
+    ierr = dispatch_build<double>(Q, build, op->num_elem, PM, d_mo, mesh_comp_stride, coords, d_mats, blk, (double *)op->d_qdata, s);
+  else ierr = dispatch_build<float>(Q, build, op->num_elem, PM, d_mo, mesh_comp_stride, coords, d_mats, blk, (float *)op->d_qdata, s);
+  MI355_HIP(hipStreamSynchronize(s));
+  hipFree(d_mo);
+  hipFree(d_mats);
+  if (ierr) return ierr;
+  if (op->qf == kQfDiffusionKappa) {
+    // kappa slot defaults to 1 until CeedMi355OperatorSetKappa
+    const int64_t Q3 = (int64_t)Q * Q * Q, n = op->num_elem * Q3;
+    if (op->scalar_type == CEED_MI355_FP64)
+      fill_slot<double><<<mi355_blocks(n, 256), 256, 0, s>>>(op->num_elem, Q3, 7 * Q3, 6 * Q3, 1.0, (double *)op->d_qdata);
+    else fill_slot<float><<<mi355_blocks(n, 256), 256, 0, s>>>(op->num_elem, Q3, 7 * Q3, 6 * Q3, 1.0f, (float *)op->d_qdata);
+    MI355_LAUNCH_CHECK();
+  }
+  return CEED_MI355_SUCCESS;
+}
+
+extern "C" int CeedMi355OperatorSetQData(CeedMi355Operator op, const void *q_data, void *stream) {
+  MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
+  const int64_t per = (int64_t)(op->qf == kQfDiffusionKappa ? 6 : op->q_comp) * op->Q * op->Q * op->Q;
+  const int64_t blk = elem_block(op);
+  hipStream_t   s   = (hipStream_t)stream;
+  MI355_CHECK(per == blk || op->qf == kQfDiffusionKappa, CEED_MI355_ERROR_BACKEND, "layout mismatch");
+  if (op->qf != kQfDiffusionKappa) {
+    if (op->scalar_type == CEED_MI355_FP64)
+      permute_elements<double, double><<<mi355_blocks(op->num_elem * per, 256), 256, 0, s>>>(op->num_elem, per, op->d_perm, 1,
+                                                                                             (const double *)q_data, (double *)op->d_qdata);
+    else
+      permute_elements<float, float><<<mi355_blocks(op->num_elem * per, 256), 256, 0, s>>>(op->num_elem, per, op->d_perm, 1,
+                                                                                           (const float *)q_data, (float *)op->d_qdata);
+    MI355_LAUNCH_CHECK();
+    return CEED_MI355_SUCCESS;
+  }
+  MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED, "SetQData for the kappa operator: build the 6 Voigt entries with BuildQData");
+}
+
+extern "C" int CeedMi355OperatorGetQData(CeedMi355Operator op, void *q_data, void *stream) {
+  MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
+  const int64_t blk = elem_block(op);
+  hipStream_t   s   = (hipStream_t)stream;
+  if (op->scalar_type == CEED_MI355_FP64)
+    permute_elements<double, double><<<mi355_blocks(op->num_elem * blk, 256), 256, 0, s>>>(op->num_elem, blk, op->d_perm, 0,
+                                                                                           (const double *)op->d_qdata, (double *)q_data);
+  else
+    permute_elements<float, float><<<mi355_blocks(op->num_elem * blk, 256), 256, 0, s>>>(op->num_elem, blk, op->d_perm, 0,
+                                                                                         (const float *)op->d_qdata, (float *)q_data);
+  MI355_LAUNCH_CHECK();
+  return CEED_MI355_SUCCESS;
+}
+
+namespace {
+// kappa [elem][Q^3] (caller order, FP64) into q_data slot 6 of each element block
+template <typename T>
+__global__ void __launch_bounds__(256) scatter_kappa(int64_t num_elem, int64_t q3, const int32_t *__restrict__ perm,
+                                                     const double *__restrict__ kappa, T *__restrict__ q_data) {
+  const int64_t n = num_elem * q3;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / q3, r = i - p * q3;
+    q_data[p * 7 * q3 + 6 * q3 + r] = (T)kappa[(int64_t)perm[p] * q3 + r];
+  }
+}
+}  // namespace
+
+extern "C" int CeedMi355OperatorSetKappa(CeedMi355Operator op, const double *kappa, void *stream) {
+  MI355_CHECK(op && op->qf == kQfDiffusionKappa, CEED_MI355_ERROR_BACKEND, "SetKappa needs a kappa-diffusion operator");
+  const int64_t q3 = (int64_t)op->Q * op->Q * op->Q;
+  hipStream_t   s  = (hipStream_t)stream;
+  if (op->scalar_type == CEED_MI355_FP64)
+    scatter_kappa<double><<<mi355_blocks(op->num_elem * q3, 256), 256, 0, s>>>(op->num_elem, q3, op->d_perm, kappa, (double *)op->d_qdata);
+  else scatter_kappa<float><<<mi355_blocks(op->num_elem * q3, 256), 256, 0, s>>>(op->num_elem, q3, op->d_perm, kappa, (float *)op->d_qdata);
+  MI355_LAUNCH_CHECK();
+  return CEED_MI355_SUCCESS;
+}
+
+extern "C" int CeedMi355OperatorApplyAdd(CeedMi355Operator op, const void *u, void *v, void *stream) {
+  MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
+  if (op->num_elem == 0) return CEED_MI355_SUCCESS;
+  return op->apply(op, u, v, (hipStream_t)stream);
+}
+
+extern "C" int CeedMi355OperatorApply(CeedMi355Operator op, const void *u, void *v, void *stream) {
+  MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
+  hipStream_t s = (hipStream_t)stream;
+  if (op->scalar_type == CEED_MI355_FP64) fill_kernel<double><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, 0.0, (double *)v);
+  else fill_kernel<float><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, 0.0f, (float *)v);
+  MI355_LAUNCH_CHECK();
+  return CeedMi355OperatorApplyAdd(op, u, v, stream);
+}
+
+extern "C" int CeedMi355OperatorGetColors(CeedMi355Operator op, int32_t *num_colors, int32_t *color_sizes_host) {
+  MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
+  *num_colors = (int32_t)op->color_start.size() - 1;
+  if (color_sizes_host)
+    for (int c = 0; c < *num_colors; c++) color_sizes_host[c] = (int32_t)(op->color_start[c + 1] - op->color_start[c]);
+  return CEED_MI355_SUCCESS;
+}
+
+extern "C" int CeedMi355OperatorDestroy(CeedMi355Operator *op) {
+  if (!op || !*op) return CEED_MI355_SUCCESS;
+  hipFree((*op)->d_offsets);
+  hipFree((*op)->d_perm);
+  hipFree((*op)->d_mats);
+  hipFree((*op)->d_qdata);
+  delete *op;
+  *op = nullptr;
+  return CEED_MI355_SUCCESS;
+}

@@ -1,0 +1,92 @@
+"""Synthetic structured hex meshes on the unit cube (the benchmark inputs).
+
+``n`` elements per direction, element ``e = ex + n*(ey + n*ez)``, local node
+``i + P*(j + P*k)`` (x fastest), global node numbering lexicographic on the
+``(n*p + 1)^3`` grid, as the Cartesian restrictions of
+examples/ceed/ex1-volume.c:356-400 build them.  The geometry is trilinear
+(mesh basis P = 2) with interior vertices moved by a seeded uniform
+perturbation of +-``perturb`` * h per coordinate; boundary vertices stay put,
+so the domain is exactly the unit cube.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class HexMesh:
+    n: int                     # elements per direction
+    p: int                     # solution polynomial degree (P = p + 1)
+    num_elem: int
+    num_nodes: int             # scalar L-vector length, (n*p+1)^3
+    offsets: np.ndarray        # int32 [num_elem, P^3], solution restriction
+    mesh_offsets: np.ndarray   # int32 [num_elem, 8], trilinear geometry restriction
+    coords: np.ndarray         # float64 [3 * (n+1)^3], component stride (n+1)^3
+    num_vertices: int
+
+    @property
+    def P(self) -> int:
+        return self.p + 1
+
+
+def _lex_offsets(n: int, p: int) -> np.ndarray:
+    P, M = p + 1, n * p + 1
+    e = np.arange(n)
+    ex, ey, ez = np.meshgrid(e, e, e, indexing="ij")           # [ex, ey, ez]
+    ex, ey, ez = (a.transpose(2, 1, 0).reshape(-1) for a in (ex, ey, ez))  # element index x fastest
+    i = np.arange(P)
+    li, lj, lk = np.meshgrid(i, i, i, indexing="ij")
+    li, lj, lk = (a.transpose(2, 1, 0).reshape(-1) for a in (li, lj, lk))  # local node x fastest
+    gx = ex[:, None] * p + li[None, :]
+    gy = ey[:, None] * p + lj[None, :]
+    gz = ez[:, None] * p + lk[None, :]
+    return (gx + M * (gy + M * gz)).astype(np.int32)
+
+
+def hex_mesh(n: int, p: int, seed: int = 0, perturb: float = 0.1) -> HexMesh:
+    """Structured n^3 hex mesh of degree p on [0,1]^3 with perturbed interior vertices."""
+    if n < 1 or p < 1:
+        raise ValueError("need n >= 1 and p >= 1")
+    nv1 = n + 1
+    h = 1.0 / n
+    g = np.linspace(0.0, 1.0, nv1)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    X, Y, Z = (a.transpose(2, 1, 0).reshape(-1).copy() for a in (X, Y, Z))
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(-1.0, 1.0, size=(3, nv1 ** 3)) * perturb * h
+    idx = np.arange(nv1)
+    iz, iy, ix = np.meshgrid(idx, idx, idx, indexing="ij")
+    interior = ((ix > 0) & (ix < n) & (iy > 0) & (iy < n) & (iz > 0) & (iz < n)).reshape(-1)
+    for a, dd in zip((X, Y, Z), d):
+        a[interior] += dd[interior]
+    coords = np.concatenate([X, Y, Z]).astype(np.float64)
+    return HexMesh(n=n, p=p, num_elem=n ** 3, num_nodes=(n * p + 1) ** 3, offsets=_lex_offsets(n, p),
+                   mesh_offsets=_lex_offsets(n, 1), coords=coords, num_vertices=nv1 ** 3)
+
+
+def node_coordinates(mesh: HexMesh, gll_nodes: np.ndarray) -> np.ndarray:
+    """Physical coordinates [3, num_nodes] of the solution nodes (trilinear map of the GLL points)."""
+    P = mesh.P
+    r = (np.asarray(gll_nodes) + 1.0) / 2.0          # reference [0, 1]
+    rx, ry, rz = np.meshgrid(r, r, r, indexing="ij")
+    rx, ry, rz = (a.transpose(2, 1, 0).reshape(-1) for a in (rx, ry, rz))  # local node x fastest
+    # trilinear shape functions of the 8 vertices (local vertex a + 2*(b + 2*c))
+    shp = np.empty((P ** 3, 8))
+    for c in range(2):
+        for b in range(2):
+            for a in range(2):
+                shp[:, a + 2 * (b + 2 * c)] = (rx if a else 1 - rx) * (ry if b else 1 - ry) * (rz if c else 1 - rz)
+    nv = mesh.num_vertices
+    out = np.empty((3, mesh.num_nodes))
+    for d in range(3):
+        vc = mesh.coords[d * nv:(d + 1) * nv][mesh.mesh_offsets]   # [E, 8]
+        out[d, mesh.offsets.reshape(-1)] = (vc @ shp.T).reshape(-1)
+    return out
+
+
+def mesh_size_for_dofs(p: int, target_dofs: float) -> int:
+    """Elements per direction giving about `target_dofs` scalar nodes at degree p."""
+    return max(1, int(round((target_dofs ** (1.0 / 3.0) - 1.0) / p)))

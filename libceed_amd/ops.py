@@ -1,0 +1,214 @@
+"""Python objects over the C ABI: restriction, tensor basis, fused hex operator.
+
+Device arrays are torch CUDA (HIP) tensors; every call is enqueued on torch's
+current stream.  These wrappers are what the tests, the benchmark and the
+autograd bindings use; the computation always runs in the HIP kernels of
+``libceed-mi355.so``.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call
+
+NOTRANSPOSE, TRANSPOSE = 0, 1
+EVAL_NONE, EVAL_INTERP, EVAL_GRAD, EVAL_WEIGHT = 0, 1, 2, 16
+QF_MASS, QF_DIFFUSION, QF_DIFFUSION_KAPPA = 0, 1, 2
+FP64, FP32 = 0, 1
+GAUSS, GAUSS_LOBATTO = 0, 1
+
+
+def _ptr(a) -> C.c_void_p:
+    if a is None:
+        return C.c_void_p(0)
+    if isinstance(a, torch.Tensor):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return C.c_void_p(a.data_ptr())
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return C.c_void_p(a.ctypes.data)
+    raise TypeError(type(a))
+
+
+def _stream() -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _device_tensor(a, dtype) -> torch.Tensor:
+    if isinstance(a, torch.Tensor):
+        if a.device.type != "cuda":
+            raise ValueError("expected a device tensor")
+        if a.dtype != dtype:
+            raise TypeError(f"expected {dtype}, got {a.dtype}")
+        return a
+    raise TypeError("expected a torch CUDA tensor")
+
+
+def device_init(device_id: int = 0) -> None:
+    call("CeedMi355DeviceInit", device_id)
+
+
+def lagrange_basis_1d(P: int, Q: int, quad_mode: int = GAUSS):
+    """(interp, grad, q_ref, q_weight) of the 1D Lagrange basis on P GLL nodes at Q points (host, FP64)."""
+    interp = np.zeros((Q, P))
+    grad = np.zeros((Q, P))
+    qref = np.zeros(Q)
+    qw = np.zeros(Q)
+    call("CeedMi355LagrangeBasis1D", P, Q, quad_mode, _ptr(interp), _ptr(grad), _ptr(qref), _ptr(qw))
+    return interp, grad, qref, qw
+
+
+def gauss_quadrature(Q: int):
+    x, w = np.zeros(Q), np.zeros(Q)
+    call("CeedGaussQuadrature", Q, _ptr(x), _ptr(w))
+    return x, w
+
+
+def lobatto_quadrature(Q: int):
+    x, w = np.zeros(Q), np.zeros(Q)
+    call("CeedLobattoQuadrature", Q, _ptr(x), _ptr(w))
+    return x, w
+
+
+class Restriction:
+    """Element restriction E (offsets or strides) with bit-exact transpose."""
+
+    def __init__(self, num_elem: int, elem_size: int, num_comp: int, l_size: int, offsets=None, comp_stride: int = 1,
+                 strides=None):
+        self.num_elem, self.elem_size, self.num_comp, self.l_size = num_elem, elem_size, num_comp, l_size
+        h = C.c_void_p()
+        if offsets is not None:
+            off = np.ascontiguousarray(offsets, dtype=np.int32).reshape(-1)
+            if off.size != num_elem * elem_size:
+                raise ValueError("offsets must have num_elem * elem_size entries")
+            call("CeedMi355RestrictionCreateOffset", num_elem, elem_size, num_comp, comp_stride, l_size, _ptr(off), C.byref(h))
+        else:
+            st = np.ascontiguousarray(strides, dtype=np.int32)
+            call("CeedMi355RestrictionCreateStrided", num_elem, elem_size, num_comp, l_size, _ptr(st), C.byref(h))
+        self._h = h
+
+    @property
+    def e_size(self) -> int:
+        return self.num_elem * self.elem_size * self.num_comp
+
+    def apply(self, t_mode: int, u: torch.Tensor, v: torch.Tensor) -> None:
+        u, v = _device_tensor(u, torch.float64), _device_tensor(v, torch.float64)
+        need_u, need_v = (self.l_size, self.e_size) if t_mode == NOTRANSPOSE else (self.e_size, self.l_size)
+        if u.numel() < need_u or v.numel() < need_v:
+            raise ValueError(f"vector too short for restriction ({u.numel()}, {v.numel()}) vs ({need_u}, {need_v})")
+        call("CeedMi355RestrictionApply", self._h, t_mode, _ptr(u), _ptr(v), _stream())
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib.lib().CeedMi355RestrictionDestroy(C.byref(self._h))
+
+
+class Basis:
+    """Tensor-product H1 basis of dimension 1..3."""
+
+    def __init__(self, dim: int, num_comp: int, P: int, Q: int, interp=None, grad=None, q_weight=None, quad_mode: int = GAUSS):
+        if interp is None:
+            interp, grad, _, q_weight = lagrange_basis_1d(P, Q, quad_mode)
+        self.dim, self.num_comp, self.P, self.Q = dim, num_comp, P, Q
+        self.interp = np.ascontiguousarray(interp, dtype=np.float64)
+        self.grad = np.ascontiguousarray(grad, dtype=np.float64)
+        self.q_weight = np.ascontiguousarray(q_weight, dtype=np.float64)
+        h = C.c_void_p()
+        call("CeedMi355BasisCreateTensorH1", dim, num_comp, P, Q, _ptr(self.interp), _ptr(self.grad), _ptr(self.q_weight), C.byref(h))
+        self._h = h
+
+    def apply(self, num_elem: int, t_mode: int, eval_mode: int, u, v: torch.Tensor, add: bool = False) -> None:
+        v = _device_tensor(v, torch.float64)
+        if u is not None:
+            u = _device_tensor(u, torch.float64)
+        call("CeedMi355BasisApply", self._h, num_elem, t_mode, eval_mode, int(add), _ptr(u), _ptr(v), _stream())
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib.lib().CeedMi355BasisDestroy(C.byref(self._h))
+
+
+class HexOperator:
+    """Fused 3D hex operator v = E^T B^T D B E u with a gallery pointwise D.
+
+    qf: QF_MASS, QF_DIFFUSION or QF_DIFFUSION_KAPPA; dtype torch.float64 or
+    torch.float32 (FP32 storage and arithmetic for u, v and q_data).
+    """
+
+    def __init__(self, qf: int, offsets: np.ndarray, num_elem: int, P: int, Q: int, l_size: int, num_comp: int = 1,
+                 comp_stride: int | None = None, dtype=torch.float64, quad_mode: int = GAUSS):
+        self.qf, self.P, self.Q, self.num_elem, self.num_comp = qf, P, Q, num_elem, num_comp
+        self.dtype = dtype
+        self.quad_mode = quad_mode
+        self.comp_stride = comp_stride if comp_stride is not None else (l_size // num_comp)
+        self.l_size = l_size
+        self.interp, self.grad, self.q_ref, self.q_weight = lagrange_basis_1d(P, Q, quad_mode)
+        off = np.ascontiguousarray(offsets, dtype=np.int32).reshape(-1)
+        h = C.c_void_p()
+        st = FP64 if dtype == torch.float64 else FP32
+        call("CeedMi355OperatorCreateHex", qf, st, num_comp, num_elem, P, Q, _ptr(self.interp), _ptr(self.grad), _ptr(off),
+             self.comp_stride, l_size, C.byref(h))
+        self._h = h
+        self.q_comp = {QF_MASS: 1, QF_DIFFUSION: 6, QF_DIFFUSION_KAPPA: 7}[qf]
+
+    @classmethod
+    def from_mesh(cls, qf: int, mesh, Q: int | None = None, num_comp: int = 1, dtype=torch.float64, build: bool = True):
+        """Operator on a HexMesh with geometric factors built on the GPU from its coordinates."""
+        Q = Q if Q is not None else mesh.P + 1
+        op = cls(qf, mesh.offsets, mesh.num_elem, mesh.P, Q, num_comp * mesh.num_nodes, num_comp=num_comp,
+                 comp_stride=mesh.num_nodes, dtype=dtype)
+        if build:
+            coords = torch.from_numpy(mesh.coords).to("cuda")
+            op.build_qdata(mesh.mesh_offsets, coords, mesh.num_vertices, mesh_P=2)
+        return op
+
+    def build_qdata(self, mesh_offsets: np.ndarray, coords: torch.Tensor, mesh_comp_stride: int, mesh_P: int = 2) -> None:
+        mi, mg, _, mw = lagrange_basis_1d(mesh_P, self.Q, self.quad_mode)
+        mo = np.ascontiguousarray(mesh_offsets, dtype=np.int32).reshape(-1)
+        coords = _device_tensor(coords, torch.float64)
+        build = 0 if self.qf == QF_MASS else 1
+        call("CeedMi355OperatorBuildQData", self._h, build, mesh_P, _ptr(mi), _ptr(mg), _ptr(mw), _ptr(mo), mesh_comp_stride,
+             _ptr(coords), _stream())
+
+    def set_qdata(self, q_data: torch.Tensor) -> None:
+        call("CeedMi355OperatorSetQData", self._h, _ptr(_device_tensor(q_data, self.dtype)), _stream())
+
+    def get_qdata(self) -> torch.Tensor:
+        out = torch.empty(self.num_elem * self.q_comp * self.Q ** 3, dtype=self.dtype, device="cuda")
+        call("CeedMi355OperatorGetQData", self._h, _ptr(out), _stream())
+        return out
+
+    def set_kappa(self, kappa: torch.Tensor) -> None:
+        call("CeedMi355OperatorSetKappa", self._h, _ptr(_device_tensor(kappa, torch.float64)), _stream())
+
+    def apply(self, u: torch.Tensor, v: torch.Tensor) -> None:
+        self._check(u, v)
+        call("CeedMi355OperatorApply", self._h, _ptr(u), _ptr(v), _stream())
+
+    def apply_add(self, u: torch.Tensor, v: torch.Tensor) -> None:
+        self._check(u, v)
+        call("CeedMi355OperatorApplyAdd", self._h, _ptr(u), _ptr(v), _stream())
+
+    def colors(self) -> list[int]:
+        n = C.c_int32()
+        call("CeedMi355OperatorGetColors", self._h, C.byref(n), C.c_void_p(0))
+        sizes = np.zeros(n.value, dtype=np.int32)
+        call("CeedMi355OperatorGetColors", self._h, C.byref(n), _ptr(sizes))
+        return sizes.tolist()
+
+    def _check(self, u, v):
+        for a in (u, v):
+            _device_tensor(a, self.dtype)
+            if a.numel() < self.l_size:
+                raise ValueError(f"L-vector has {a.numel()} entries, operator needs {self.l_size}")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib.lib().CeedMi355OperatorDestroy(C.byref(self._h))

@@ -1,0 +1,237 @@
+"""GPU parity of the HIP kernels against the oracle (/cpu/self/ref/serial
+restatement) and the reference's golden vectors.
+
+Bar: restriction E and E^T bit-exact; FP64 basis and operator within
+1e-12 relative L2 (FP64 rounding differences from FMA and summation order);
+FP32 operator within 2e-5 relative L2 of the FP64 oracle.
+"""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from libceed_amd import ops
+from libceed_amd.mesh import hex_mesh, node_coordinates
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+RTOL64 = 1e-12
+RTOL32 = 2e-5
+
+
+def dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device="cuda", dtype=dtype)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.double().cpu().numpy()
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+# ------------------------------------------------------------ restriction --
+def test_restriction_golden_bitexact(cuda):
+    g = np.load(GOLD / "golden_rstr.npz", allow_pickle=False)
+    for name in sorted({k.rsplit("_", 1)[0] for k in g.files}):
+        E, es, nc, cs, ls = (int(x) for x in g[name + "_meta"])
+        print("restriction golden", name, flush=True)
+        r = ops.Restriction(E, es, nc, ls, offsets=g[name + "_off"], comp_stride=cs)
+        e = torch.zeros(E * es * nc, dtype=torch.float64, device="cuda")
+        r.apply(ops.NOTRANSPOSE, dev(g[name + "_u"]), e)
+        assert np.array_equal(host(e), g[name + "_e"]), name
+        v = dev(g[name + "_v0"])
+        r.apply(ops.TRANSPOSE, e, v)
+        assert np.array_equal(host(v), g[name + "_v"]), name
+
+
+@pytest.mark.parametrize("nc", [1, 3])
+def test_restriction_mesh_bitexact(cuda, nc):
+    m = hex_mesh(6, 4, seed=2)
+    N = m.num_nodes
+    rng = np.random.default_rng(nc)
+    u, v0 = rng.uniform(-1, 1, (2, nc * N))
+    r = ops.Restriction(m.num_elem, m.P ** 3, nc, nc * N, offsets=m.offsets, comp_stride=N)
+    e = torch.zeros(r.e_size, dtype=torch.float64, device="cuda")
+    r.apply(ops.NOTRANSPOSE, dev(u), e)
+    e_ref = np.zeros(r.e_size)
+    oracle.restriction_offset(m.num_elem, m.P ** 3, nc, N, m.offsets, 0, u, e_ref)
+    assert np.array_equal(host(e), e_ref)
+    ev = rng.uniform(-1, 1, r.e_size)          # arbitrary E-vector, nonzero start vector
+    v = dev(v0)
+    r.apply(ops.TRANSPOSE, dev(ev), v)
+    v_ref = v0.copy()
+    oracle.restriction_offset(m.num_elem, m.P ** 3, nc, N, m.offsets, 1, ev, v_ref)
+    assert np.array_equal(host(v), v_ref)
+
+
+def test_restriction_strided_and_empty(cuda):
+    E, es, nc = 5, 8, 2
+    strides = np.array([1, es, es * nc], dtype=np.int32)
+    rng = np.random.default_rng(4)
+    u = rng.uniform(-1, 1, E * es * nc)
+    r = ops.Restriction(E, es, nc, E * es * nc, strides=strides)
+    e = torch.zeros(E * es * nc, dtype=torch.float64, device="cuda")
+    r.apply(ops.NOTRANSPOSE, dev(u), e)
+    e_ref = np.zeros_like(u)
+    oracle.restriction_strided(E, es, nc, strides, 0, u, e_ref)
+    assert np.array_equal(host(e), e_ref)
+    v0 = rng.uniform(-1, 1, u.size)
+    v = dev(v0)
+    r.apply(ops.TRANSPOSE, e, v)
+    oracle.restriction_strided(E, es, nc, strides, 1, e_ref, v0)
+    assert np.array_equal(host(v), v0)
+    empty = ops.Restriction(0, 4, 1, 3, offsets=np.zeros(0, dtype=np.int32))
+    z = torch.zeros(3, dtype=torch.float64, device="cuda")
+    empty.apply(ops.TRANSPOSE, torch.zeros(1, dtype=torch.float64, device="cuda"), z)
+    assert not host(z).any()
+
+
+# ------------------------------------------------------------------ basis --
+def test_basis_golden(cuda):
+    g = np.load(GOLD / "golden_bapply.npz", allow_pickle=False)
+    n = 0
+    for k in sorted(g.files):
+        if not k.endswith("_u"):
+            continue
+        key = k[:-2]
+        dim, nc, P, Q, tm, em = (int(s[1:]) for s in key.split("_"))
+        b = ops.Basis(dim, nc, P, Q)
+        ref = g[key + "_v"]
+        v = torch.zeros(ref.size, dtype=torch.float64, device="cuda")
+        b.apply(1, tm, em, None if em == 16 else dev(g[k]), v)
+        err = rel_l2(host(v), ref)
+        assert err < RTOL64, (key, err)
+        n += 1
+    assert n > 100
+
+
+@pytest.mark.parametrize("dim,P,Q", [(3, 5, 6), (3, 4, 3), (2, 3, 3), (1, 6, 8)])
+def test_basis_many_elements(cuda, dim, P, Q):
+    """Element-major layout over several elements against the oracle one element at a time."""
+    nc, E = 2, 7
+    B, G, _, w = oracle.lagrange_1d(P, Q)
+    rng = np.random.default_rng(dim * 100 + P)
+    nn, nq = P ** dim, Q ** dim
+    b = ops.Basis(dim, nc, P, Q)
+    u = rng.uniform(-1, 1, E * nc * nn)
+    for em in (ops.EVAL_INTERP, ops.EVAL_GRAD):
+        qs = nc * nq * (dim if em == ops.EVAL_GRAD else 1)
+        v = torch.zeros(E * qs, dtype=torch.float64, device="cuda")
+        b.apply(E, ops.NOTRANSPOSE, em, dev(u), v)
+        ref = np.concatenate([oracle.basis_apply(dim, nc, B, G, w, 1, 0, em, u[e * nc * nn:(e + 1) * nc * nn]) for e in range(E)])
+        assert rel_l2(host(v), ref) < RTOL64
+        r = rng.uniform(-1, 1, E * qs)
+        vt0 = rng.uniform(-1, 1, E * nc * nn)
+        vt = dev(vt0)
+        b.apply(E, ops.TRANSPOSE, em, dev(r), vt, add=True)
+        reft = vt0 + np.concatenate([oracle.basis_apply(dim, nc, B, G, w, 1, 1, em, r[e * qs:(e + 1) * qs]) for e in range(E)])
+        assert rel_l2(host(vt), reft) < RTOL64
+
+
+# --------------------------------------------------------- fused operator --
+def golden_op_cases():
+    g = np.load(GOLD / "golden_op.npz", allow_pickle=False)
+    return sorted(k[:-5] for k in g.files if k.endswith("_meta"))
+
+
+@pytest.mark.parametrize("key", golden_op_cases())
+def test_operator_golden(cuda, key):
+    g = np.load(GOLD / "golden_op.npz", allow_pickle=False)
+    kind, nc, n, p, Q, seed = (int(x) for x in g[key + "_meta"])
+    print("operator golden", key, flush=True)
+    m = hex_mesh(n, p, seed=seed)
+    op = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc)
+    qd = host(op.get_qdata())
+    gq = g[key + "_qdata"]
+    assert rel_l2(qd[:gq.size], gq) < RTOL64
+    v = torch.zeros(nc * m.num_nodes, dtype=torch.float64, device="cuda")
+    op.apply(dev(g[key + "_u"]), v)
+    assert rel_l2(host(v), g[key + "_v"]) < RTOL64
+
+
+@pytest.mark.parametrize("kind,nc,n,p,Q", [(1, 1, 5, 4, 6), (1, 3, 4, 3, 5), (0, 1, 5, 6, 8), (1, 1, 3, 8, 10), (1, 1, 4, 4, 5),
+                                           (0, 3, 3, 2, 3), (1, 1, 6, 1, 2)])
+def test_operator_vs_oracle(cuda, kind, nc, n, p, Q):
+    m = hex_mesh(n, p, seed=n + p)
+    rng = np.random.default_rng(p)
+    u = rng.uniform(-1, 1, nc * m.num_nodes)
+    qd_ref = oracle.hex_build_qdata(kind, m, Q)
+    v_ref = oracle.hex_apply(kind, nc, m.num_elem, p + 1, Q, m.offsets, m.num_nodes, qd_ref, u)
+    # geometric factors built on the GPU
+    op = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc)
+    assert rel_l2(host(op.get_qdata()), qd_ref) < RTOL64
+    v = torch.zeros_like(dev(u))
+    op.apply(dev(u), v)
+    assert rel_l2(host(v), v_ref) < RTOL64
+    # geometric factors handed over in the CEED_STRIDES_BACKEND layout
+    op2 = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, build=False)
+    op2.set_qdata(dev(qd_ref))
+    v2 = torch.zeros_like(v)
+    op2.apply(dev(u), v2)
+    assert rel_l2(host(v2), v_ref) < RTOL64
+    # ApplyAdd sums into the output
+    w0 = rng.uniform(-1, 1, u.size)
+    w = dev(w0)
+    op2.apply_add(dev(u), w)
+    assert rel_l2(host(w), w0 + v_ref) < RTOL64
+
+
+def test_mass_config_repeated_apply(cuda):
+    """Mass operator, 16^3 elements, p = 3, Q = 5: v = M u applied 100 times, bitwise repeatable, oracle parity."""
+    m = hex_mesh(16, 3, seed=1)
+    u = np.random.default_rng(3).uniform(-1, 1, m.num_nodes)
+    qd = oracle.hex_build_qdata(0, m, 5)
+    v_ref = oracle.hex_apply(0, 1, m.num_elem, 4, 5, m.offsets, m.num_nodes, qd, u)
+    op = ops.HexOperator.from_mesh(ops.QF_MASS, m, Q=5)
+    ud, v = dev(u), torch.zeros(m.num_nodes, dtype=torch.float64, device="cuda")
+    op.apply(ud, v)
+    first = v.clone()
+    for _ in range(99):
+        op.apply(ud, v)
+    assert torch.equal(first, v)
+    assert rel_l2(host(v), v_ref) < RTOL64
+
+
+@pytest.mark.parametrize("kind,p", [(1, 4), (1, 6), (0, 4)])
+def test_operator_fp32(cuda, kind, p):
+    m = hex_mesh(4, p, seed=11)
+    Q = p + 2
+    u = np.random.default_rng(0).uniform(-1, 1, m.num_nodes)
+    v_ref = oracle.hex_apply(kind, 1, m.num_elem, p + 1, Q, m.offsets, m.num_nodes, oracle.hex_build_qdata(kind, m, Q), u)
+    op = ops.HexOperator.from_mesh(kind, m, Q=Q, dtype=torch.float32)
+    v = torch.zeros(m.num_nodes, dtype=torch.float32, device="cuda")
+    op.apply(dev(u, torch.float32), v)
+    assert rel_l2(host(v), v_ref) < RTOL32
+
+
+@pytest.mark.parametrize("p", [2, 4, 7])
+def test_operator_invariants_large(cuda, p):
+    """Size-independent properties at ~0.5-1M DoFs: volume, A 1 = 0, int |grad(x+y+z)|^2 = 3, symmetry."""
+    n = max(2, int(round((8e5 ** (1 / 3) - 1) / p)))
+    m = hex_mesh(n, p, seed=21)
+    Q = p + 2
+    A = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q)
+    M = ops.HexOperator.from_mesh(ops.QF_MASS, m, Q=Q)
+    one = torch.ones(m.num_nodes, dtype=torch.float64, device="cuda")
+    v = torch.empty_like(one)
+    M.apply(one, v)
+    assert abs(v.sum().item() - 1.0) < 1e-11
+    A.apply(one, v)
+    assert v.abs().max().item() < 1e-11
+    gll, _ = oracle.lobatto(p + 1)
+    u = dev(node_coordinates(m, gll).sum(0))
+    A.apply(u, v)
+    assert abs(torch.dot(u, v).item() - 3.0) < 1e-9
+    g = torch.Generator(device="cuda").manual_seed(p)
+    a = torch.rand(m.num_nodes, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    b = torch.rand(m.num_nodes, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    Aa, Ab = torch.empty_like(a), torch.empty_like(a)
+    A.apply(a, Aa)
+    A.apply(b, Ab)
+    assert abs(torch.dot(a, Ab).item() - torch.dot(b, Aa).item()) < 1e-10 * abs(torch.dot(a, Ab).item())
