@@ -145,6 +145,8 @@ def main() -> None:
     ap.add_argument("--dofs", type=float, default=8e6)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="time a plain 1 GiB device copy (HBM reference for bandwidth and counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it (see DESIGN.md, sharding)"}))
@@ -155,6 +157,14 @@ def main() -> None:
     from libceed_amd import ops
 
     ops.device_init(0)
+    if args.copy_calibration:
+        nbytes = 1 << 30
+        a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        ms, _ = time_gpu(lambda: b.copy_(a), args.steps, args.warmup)
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "ms_per_copy": ms,
+                          "GBs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return
     head = run_case(args.p, args.dofs, torch.float64, args.steps, args.warmup)
     sweep = {}
     if not args.no_sweep:
