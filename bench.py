@@ -136,6 +136,19 @@ def cpu_baseline(p: int = 4, seconds: float = 10.0) -> dict | None:
             "sample": f"oracle (ref/serial restatement), p={p}, 8^3 elements, {dt:.0f} s"}
 
 
+def measured_traffic(p: int, dofs: int, dtype: str):
+    """HBM bytes per apply from the committed rocprofv3 counter summary of this workload, if one exists."""
+    f = ROOT / "profiles" / f"hex_apply_p{p}_8M_{dtype}_traffic.json"
+    try:
+        d = json.loads(f.read_text())
+    except (OSError, ValueError):
+        return None
+    w = d.get("workload", {})
+    if w.get("p") == p and w.get("dofs") == dofs and w.get("dtype") == dtype:
+        return d["traffic_bytes_per_apply"]
+    return None
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,7 +205,7 @@ def main() -> None:
                                f"{head['n']}^3 hex elements, {head['dofs']} DoFs",
                    "p": args.p, "dofs": head["dofs"], "elements": head["elements"], "element_colors": head["colors"]},
         "roofline": {"bound": "hbm", "achieved": round(head["kernel_GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(head["hbm_frac"], 4), "traffic": None,
+                     "frac": round(head["hbm_frac"], 4), "traffic": measured_traffic(args.p, head["dofs"], "fp64"),
                      "kernel": "hex_apply_kernel (all colour launches of one apply)",
                      "alg_bytes_per_apply": head["alg_bytes"], "kernel_ms_per_apply": round(head["kernel_ms"], 5),
                      "fp64_tflops": round(head["kernel_TFLOPs"], 2), "fp64_peak_tflops": FP64_PEAK_TFLOPS},

@@ -9,7 +9,7 @@
 // matrices B (Q x P) and D = G pinv(B) (Q x Q) are staged once per workgroup.
 //
 // Data layout in HBM (colour order: elements of colour c are contiguous):
-//   offsets [elem][P^3]   int32
+//   offsets [elem][P^3]   int32, 2 * node + first_touch (first element, in launch order, reaching the node)
 //   q_data  [elem][QC][Q^3]   QC = 1 (mass), 6 (diffusion), 7 (6 + kappa)
 //   u, v    L-vectors, component k of node n at n + k*comp_stride
 #pragma once
@@ -50,18 +50,29 @@ __host__ __device__ constexpr int hex_lds_entries(int epb) {
 
 // Pointwise Poisson3DApply (jit-source/gallery/ceed-poisson3dapply.h):
 // vg_j = sum_i ug_i S_ij with S in Voigt order {xx, yy, zz, yz, xz, xy}.
+// q_data is streamed once per apply: non-temporal loads keep it from displacing u and v in the caches.
 template <typename T>
 __device__ __forceinline__ void poisson_point(const T *qd, int64_t stride, T dx, T dy, T dz, T &vx, T &vy, T &vz) {
-  const T s0 = qd[0], s1 = qd[stride], s2 = qd[2 * stride], s3 = qd[3 * stride], s4 = qd[4 * stride], s5 = qd[5 * stride];
+  const T s0 = __builtin_nontemporal_load(qd), s1 = __builtin_nontemporal_load(qd + stride), s2 = __builtin_nontemporal_load(qd + 2 * stride),
+          s3 = __builtin_nontemporal_load(qd + 3 * stride), s4 = __builtin_nontemporal_load(qd + 4 * stride),
+          s5 = __builtin_nontemporal_load(qd + 5 * stride);
   vx = dx * s0 + dy * s5 + dz * s4;
   vy = dx * s5 + dy * s1 + dz * s3;
   vz = dx * s4 + dy * s3 + dz * s2;
 }
 
+// Workgroup -> element chunk such that the chunks one XCD runs (workgroups are
+// dealt to the 8 XCDs round-robin) are contiguous in the element order.
+__device__ __forceinline__ int64_t xcd_chunk(int64_t b, int64_t grid) {
+  const int64_t x = b % 8, k = b / 8, q = grid / 8, r = grid % 8;
+  return x * q + (x < r ? x : r) + k;
+}
+
 template <int P, int Q, int NCOMP, int QF, typename T>
-__global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int64_t elem_end, const int32_t *__restrict__ offsets,
-                                                         int32_t comp_stride, const T *__restrict__ q_data, const T *__restrict__ mats,
-                                                         const T *__restrict__ u, T *__restrict__ v) {
+__global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int64_t elem_end, int overwrite,
+                                                         const int32_t *__restrict__ offsets, int32_t comp_stride,
+                                                         const T *__restrict__ q_data, const T *__restrict__ mats, const T *__restrict__ u,
+                                                         T *__restrict__ v) {
   using S = HexShape<P, Q>;
   constexpr int T1 = S::T1, NT = S::NT, SLAB = S::SLAB, P3 = S::P3, Q3 = S::Q3, QC = QfTraits<QF>::kComps;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -76,12 +87,12 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
   T        *S2 = S1 + SLAB;  // only used by the diffusion variants
   for (int i = threadIdx.x; i < Q * P + Q * Q; i += blockDim.x) sB[i] = mats[i];
 
-  const int64_t e    = elem_begin + (int64_t)blockIdx.x * epb + le;
+  const int64_t e    = elem_begin + xcd_chunk(blockIdx.x, gridDim.x) * epb + le;
   const bool    live = le < epb && e < elem_end;
   const bool    node = live && tx < P && ty < P;
   auto sidx = [](int z, int y, int x) { return (z * T1 + y) * T1 + x; };
 
-  int off[P];
+  int off[P];  // 2 * node + first_touch
   if (node) {
 #pragma unroll
     for (int z = 0; z < P; z++) off[z] = offsets[e * P3 + (z * P + ty) * P + tx];
@@ -94,7 +105,7 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
     // ---- E: gather the element's nodes of this component into slab S0[z][y][x]
     if (node) {
 #pragma unroll
-      for (int z = 0; z < P; z++) S0[sidx(z, ty, tx)] = u[off[z] + coff];
+      for (int z = 0; z < P; z++) S0[sidx(z, ty, tx)] = u[(off[z] >> 1) + coff];
     }
     __syncthreads();
     // ---- B, x direction: S1[z][y][qx] = sum_b B[qx][b] S0[z][y][b]   (y, z < P)
@@ -136,7 +147,7 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
     if constexpr (QF == kQfMass) {
       // ---- D: v = u * (w det J)
 #pragma unroll
-      for (int qz = 0; qz < Q; qz++) uq[qz] *= live ? qd[qz * Q * Q] : T(0);
+      for (int qz = 0; qz < Q; qz++) uq[qz] *= live ? __builtin_nontemporal_load(&qd[qz * Q * Q]) : T(0);
 #pragma unroll
       for (int b = 0; b < P; b++) {
         T s = 0;
@@ -164,7 +175,7 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
           const T *q = qd + qz * Q * Q;
           poisson_point<T>(q, Q3, dx, dy, dz, vx, vy, vzz);
           if constexpr (QF == kQfDiffusionKappa) {
-            const T k = q[6 * Q3];
+            const T k = __builtin_nontemporal_load(q + 6 * Q3);
             vx *= k;
             vy *= k;
             vzz *= k;
@@ -221,8 +232,9 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
         T s = 0;
 #pragma unroll
         for (int a = 0; a < Q; a++) s += sB[a * P + tx] * S1[sidx(z, ty, a)];
-        T *dst = &v[off[z] + coff];
-        *dst += s;
+        T *dst = &v[(off[z] >> 1) + coff];
+        if (overwrite && (off[z] & 1)) *dst = s;  // first element to reach this entry: CeedOperatorApply writes
+        else *dst += s;
       }
     }
   }

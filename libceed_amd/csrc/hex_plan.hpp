@@ -13,7 +13,7 @@ struct CeedMi355Operator_private;
 
 namespace mi355 {
 
-using ApplyFn = int (*)(const CeedMi355Operator_private *op, const void *u, void *v, hipStream_t s);
+using ApplyFn = int (*)(const CeedMi355Operator_private *op, const void *u, void *v, int overwrite, hipStream_t s);
 
 // Kernel launcher for one (P, Q, num_comp, QF, scalar type); nullptr when no instantiation exists.
 ApplyFn pick_apply_f64(int P, int Q, int num_comp, int qf);
@@ -29,6 +29,7 @@ struct CeedMi355Operator_private {
   int32_t num_elem, comp_stride;
   int64_t l_size;
   int     epb;
+  bool    covers_all = false;  // every L entry is reached by some element: Apply needs no zero fill
   // colour c owns packed elements [color_start[c], color_start[c+1])
   std::vector<int64_t> color_start;
   std::vector<int32_t> perm;  // packed position -> caller's element index
@@ -42,7 +43,7 @@ struct CeedMi355Operator_private {
 namespace mi355 {
 
 template <int P, int Q, int NC, int QF, typename T>
-int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v, hipStream_t s) {
+int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v, int overwrite, hipStream_t s) {
   const int    nt  = Q * Q;
   const int    epb = op->epb;
   const size_t lds = sizeof(T) * (size_t)hex_lds_entries<P, Q, QF>(epb);
@@ -50,7 +51,7 @@ int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v
     const int64_t b = op->color_start[c], e = op->color_start[c + 1];
     if (e <= b) continue;
     const int64_t grid = (e - b + epb - 1) / epb;
-    hex_apply_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, op->d_offsets, op->comp_stride, (const T *)op->d_qdata,
+    hex_apply_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, overwrite, op->d_offsets, op->comp_stride, (const T *)op->d_qdata,
                                                                           (const T *)op->d_mats, (const T *)u, (T *)v);
   }
   MI355_LAUNCH_CHECK();

@@ -256,8 +256,36 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
     std::vector<int64_t> cursor(op->color_start.begin(), op->color_start.end() - 1);
     for (int64_t e = 0; e < num_elem; e++) op->perm[cursor[color[e]]++] = (int32_t)e;
   }
+  // offsets in colour order, encoded 2 * node + first_touch: the first element (in launch order)
+  // reaching a node may overwrite v in CeedOperatorApply instead of accumulating
   std::vector<int32_t> packed((size_t)num_elem * es);
-  for (int64_t p = 0; p < num_elem; p++) memcpy(&packed[p * es], &offsets_host[(int64_t)op->perm[p] * es], sizeof(int32_t) * es);
+  {
+    int64_t n_nodes = 0;
+    for (int64_t i = 0; i < (int64_t)num_elem * es; i++) n_nodes = std::max<int64_t>(n_nodes, (int64_t)offsets_host[i] + 1);
+    if (n_nodes >= (1ll << 30)) {
+      delete op;
+      MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED, "more than 2^30 nodes");
+    }
+    std::vector<char> touched(n_nodes, 0);
+    for (int64_t p = 0; p < num_elem; p++)
+      for (int i = 0; i < es; i++) {
+        const int32_t node = offsets_host[(int64_t)op->perm[p] * es + i];
+        packed[p * es + i] = 2 * node + (touched[node] ? 0 : 1);
+        touched[node]      = 1;
+      }
+    std::vector<uint8_t> hit(l_size, 0);
+    for (int64_t n = 0; n < n_nodes; n++)
+      if (touched[n])
+        for (int k = 0; k < num_comp; k++) {
+          uint8_t &h = hit[n + (int64_t)k * comp_stride];
+          if (h) {
+            delete op;
+            MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED, "components overlap at L index %lld", (long long)(n + (int64_t)k * comp_stride));
+          }
+          h = 1;
+        }
+    op->covers_all = std::all_of(hit.begin(), hit.end(), [](uint8_t h) { return h != 0; });
+  }
 
   // B and the collocated derivative D = G pinv(B), in the plan's scalar type
   std::vector<double> mats(Q_1d * P_1d + Q_1d * Q_1d);
@@ -383,16 +411,21 @@ extern "C" int CeedMi355OperatorSetKappa(CeedMi355Operator op, const double *kap
 extern "C" int CeedMi355OperatorApplyAdd(CeedMi355Operator op, const void *u, void *v, void *stream) {
   MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
   if (op->num_elem == 0) return CEED_MI355_SUCCESS;
-  return op->apply(op, u, v, (hipStream_t)stream);
+  return op->apply(op, u, v, 0, (hipStream_t)stream);
 }
 
 extern "C" int CeedMi355OperatorApply(CeedMi355Operator op, const void *u, void *v, void *stream) {
   MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
   hipStream_t s = (hipStream_t)stream;
-  if (op->scalar_type == CEED_MI355_FP64) fill_kernel<double><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, 0.0, (double *)v);
-  else fill_kernel<float><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, 0.0f, (float *)v);
-  MI355_LAUNCH_CHECK();
-  return CeedMi355OperatorApplyAdd(op, u, v, stream);
+  // first-touch entries are overwritten by the kernel; entries no element reaches are zeroed here
+  if (!op->covers_all || op->num_elem == 0) {
+    if (op->scalar_type == CEED_MI355_FP64)
+      fill_kernel<double><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, 0.0, (double *)v);
+    else fill_kernel<float><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, 0.0f, (float *)v);
+    MI355_LAUNCH_CHECK();
+  }
+  if (op->num_elem == 0) return CEED_MI355_SUCCESS;
+  return op->apply(op, u, v, 1, s);
 }
 
 extern "C" int CeedMi355OperatorGetColors(CeedMi355Operator op, int32_t *num_colors, int32_t *color_sizes_host) {
