@@ -137,6 +137,51 @@ int CeedMi355OperatorApply(CeedMi355Operator op, const void *u, void *v, void *s
 int CeedMi355OperatorGetColors(CeedMi355Operator op, int32_t *num_colors, int32_t *color_sizes_host);
 int CeedMi355OperatorDestroy(CeedMi355Operator *op);
 
+/* ------------------------------------------------------------------------ */
+/* L-vector arithmetic                                                       */
+/* ------------------------------------------------------------------------ */
+enum {
+  CEED_MI355_VEC_SET        = 0, /* z = alpha                       CeedVectorSetValue    (interface/ceed-vector.c:377) */
+  CEED_MI355_VEC_SCALE      = 1, /* z = alpha z                     CeedVectorScale       (interface/ceed-vector.c:735) */
+  CEED_MI355_VEC_AXPY       = 2, /* z = alpha x + z                 CeedVectorAXPY        (interface/ceed-vector.c:810) */
+  CEED_MI355_VEC_AXPBY      = 3, /* z = alpha x + beta z            CeedVectorAXPBY       (interface/ceed-vector.c:881) */
+  CEED_MI355_VEC_PMULT      = 4, /* z = x .* y                      CeedVectorPointwiseMult (interface/ceed-vector.c:953) */
+  CEED_MI355_VEC_RECIPROCAL = 5, /* z = 1/z where |z| > alpha       CeedVectorReciprocal  (interface/ceed-vector.c:1042) */
+};
+enum {
+  CEED_MI355_RED_DOT     = 0, /* sum x_i y_i  (x == y gives the squared 2-norm) */
+  CEED_MI355_RED_ABS_SUM = 1, /* sum |x_i|    (CEED_NORM_1) */
+  CEED_MI355_RED_MAX_ABS = 2, /* max |x_i|    (CEED_NORM_MAX) */
+};
+/* Elementwise vector operation in FP64 or FP32 (scalar_type).  Replaces the
+ * /cpu/self/ref/serial vector kernels (backends/ref/ceed-ref-vector.c). */
+int CeedMi355VectorOp(int op, int scalar_type, int64_t n, double alpha, double beta, const void *x, const void *y, void *z, void *stream);
+/* Deterministic reduction into one FP64 device scalar: a fixed grid and a
+ * fixed-order tree, so equal inputs give equal bits.  Replaces CeedVectorNorm
+ * (interface/ceed-vector.c:677-733) and the dot products of the CG drivers. */
+int CeedMi355VectorReduce(int op, int scalar_type, int64_t n, const void *x, const void *y, double *result_device, void *stream);
+/* y = (to_type) x, element by element (FP64 <-> FP32). */
+int CeedMi355VectorConvert(int64_t n, int from_type, const void *x, int to_type, void *y, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Run-time compiled QFunctions (hipRTC)                                     */
+/* ------------------------------------------------------------------------ */
+typedef struct CeedMi355QFunction_private *CeedMi355QFunction;
+
+/* Compile the CEED_QFUNCTION `function_name` in the source file source_path
+ * (local #include "..." files are inlined) into a pointwise kernel for
+ * gfx950.  Replaces the QFunction JIT of backends/hip-ref/ceed-hip-ref-qfunction-load.cpp:23
+ * (CeedQFunctionBuildKernel_Hip_ref). */
+int CeedMi355QFunctionCompile(const char *source_path_host, const char *function_name_host, int32_t num_in, const int32_t *in_sizes_host,
+                              int32_t num_out, const int32_t *out_sizes_host, CeedMi355QFunction *qf);
+/* Evaluate at num_elem*Q points; field i, component k of point q of element e
+ * is at in[i][(e*size_i + k)*Q + q] (in and out are host arrays of device
+ * pointers).  ctx_device is the device copy of the QFunction context or NULL.
+ * Replaces CeedQFunctionApply (interface/ceed-qfunction.c:1117). */
+int CeedMi355QFunctionApply(CeedMi355QFunction qf, int64_t num_elem, int32_t Q, void *ctx_device, const double *const *in_host,
+                            double *const *out_host, void *stream);
+int CeedMi355QFunctionDestroy(CeedMi355QFunction *qf);
+
 #ifdef __cplusplus
 }
 #endif

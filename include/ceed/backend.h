@@ -1,0 +1,125 @@
+/*
+ * ceed/backend.h -- the libCEED backend API subset a backend plugin uses:
+ * registration (CeedRegister), slot installation (CeedSetBackendFunction),
+ * per-object backend data and the getters a backend reads objects with.
+ * Names, signatures and semantics (getters return new references the caller
+ * destroys) are those of the reference's include/ceed/backend.h, so the
+ * /gpu/hip/mi355 backend source compiles against either implementation.
+ */
+#ifndef CEED_BACKEND_H
+#define CEED_BACKEND_H
+
+#include <limits.h>
+#include <stdbool.h>
+
+#include "ceed.h"
+
+#ifdef __cplusplus
+#define CEED_INTERN extern "C" __attribute__((visibility("hidden")))
+#else
+#define CEED_INTERN extern __attribute__((visibility("hidden")))
+#endif
+
+#define CEED_MAX_RESOURCE_LEN 1024
+#define CEED_MAX_BACKEND_PRIORITY UINT_MAX
+#define CEED_FIELD_MAX 16
+
+/* ---- error-checking call wrappers -------------------------------------- */
+#define CeedCall(...)                       \
+  do {                                      \
+    int ierr_q_ = __VA_ARGS__;              \
+    if (ierr_q_ != CEED_ERROR_SUCCESS) return ierr_q_; \
+  } while (0)
+/* interface errors seen from a backend become backend errors */
+#define CeedCallBackend(...)                                           \
+  do {                                                                 \
+    int ierr_q_ = __VA_ARGS__;                                         \
+    if (ierr_q_ != CEED_ERROR_SUCCESS) return ierr_q_ > 0 ? ierr_q_ : CEED_ERROR_BACKEND; \
+  } while (0)
+#define CeedCheck(cond, ceed, ecode, ...)                    \
+  do {                                                       \
+    if (!(cond)) return CeedError((ceed), (ecode), __VA_ARGS__); \
+  } while (0)
+
+/* ---- memory ------------------------------------------------------------- */
+CEED_EXTERN int CeedMallocArray(size_t n, size_t unit, void *p);
+CEED_EXTERN int CeedCallocArray(size_t n, size_t unit, void *p);
+CEED_EXTERN int CeedReallocArray(size_t n, size_t unit, void *p);
+CEED_EXTERN int CeedStringAllocCopy(const char *source, char **copy);
+CEED_EXTERN int CeedFree(void *p);
+#define CeedMalloc(n, p) CeedMallocArray((n), sizeof(**(p)), p)
+#define CeedCalloc(n, p) CeedCallocArray((n), sizeof(**(p)), p)
+#define CeedRealloc(n, p) CeedReallocArray((n), sizeof(**(p)), p)
+
+/* ---- registration and slots --------------------------------------------- */
+CEED_EXTERN int CeedRegister(const char *prefix, int (*init)(const char *, Ceed), unsigned int priority);
+CEED_EXTERN int CeedSetBackendFunctionImpl(Ceed ceed, const char *type, void *object, const char *func_name, void (*f)(void));
+#define CeedSetBackendFunction(ceed, type, object, func_name, f) CeedSetBackendFunctionImpl(ceed, type, object, func_name, (void (*)(void))f)
+CEED_EXTERN int CeedGetData(Ceed ceed, void *data);
+CEED_EXTERN int CeedSetData(Ceed ceed, void *data);
+CEED_EXTERN int CeedGetParent(Ceed ceed, Ceed *parent);
+CEED_EXTERN int CeedSetDeterministic(Ceed ceed, bool is_deterministic);
+
+/* ---- vector ------------------------------------------------------------- */
+CEED_EXTERN int CeedVectorGetCeed(CeedVector vec, Ceed *ceed);
+CEED_EXTERN int CeedVectorGetState(CeedVector vec, uint64_t *state);
+CEED_EXTERN int CeedVectorGetData(CeedVector vec, void *data);
+CEED_EXTERN int CeedVectorSetData(CeedVector vec, void *data);
+
+/* ---- element restriction ---------------------------------------------- */
+typedef enum {
+  CEED_RESTRICTION_STANDARD      = 1,
+  CEED_RESTRICTION_ORIENTED      = 2,
+  CEED_RESTRICTION_CURL_ORIENTED = 3,
+  CEED_RESTRICTION_STRIDED       = 4,
+  CEED_RESTRICTION_POINTS        = 5,
+} CeedRestrictionType;
+CEED_EXTERN int CeedElemRestrictionGetCeed(CeedElemRestriction rstr, Ceed *ceed);
+CEED_EXTERN int CeedElemRestrictionGetType(CeedElemRestriction rstr, CeedRestrictionType *rstr_type);
+CEED_EXTERN int CeedElemRestrictionIsStrided(CeedElemRestriction rstr, bool *is_strided);
+CEED_EXTERN int CeedElemRestrictionGetStrides(CeedElemRestriction rstr, CeedInt strides[3]);
+CEED_EXTERN int CeedElemRestrictionHasBackendStrides(CeedElemRestriction rstr, bool *has_backend_strides);
+CEED_EXTERN int CeedElemRestrictionGetOffsets(CeedElemRestriction rstr, CeedMemType mem_type, const CeedInt **offsets);
+CEED_EXTERN int CeedElemRestrictionRestoreOffsets(CeedElemRestriction rstr, const CeedInt **offsets);
+CEED_EXTERN int CeedElemRestrictionGetELayout(CeedElemRestriction rstr, CeedInt layout[3]);
+CEED_EXTERN int CeedElemRestrictionSetELayout(CeedElemRestriction rstr, CeedInt layout[3]);
+CEED_EXTERN int CeedElemRestrictionGetData(CeedElemRestriction rstr, void *data);
+CEED_EXTERN int CeedElemRestrictionSetData(CeedElemRestriction rstr, void *data);
+
+/* ---- basis ------------------------------------------------------------- */
+CEED_EXTERN int CeedBasisGetCeed(CeedBasis basis, Ceed *ceed);
+CEED_EXTERN int CeedBasisIsTensor(CeedBasis basis, bool *is_tensor);
+CEED_EXTERN int CeedBasisGetData(CeedBasis basis, void *data);
+CEED_EXTERN int CeedBasisSetData(CeedBasis basis, void *data);
+
+/* ---- QFunction --------------------------------------------------------- */
+CEED_EXTERN int CeedQFunctionGetCeed(CeedQFunction qf, Ceed *ceed);
+CEED_EXTERN int CeedQFunctionGetFields(CeedQFunction qf, CeedInt *num_input_fields, CeedQFunctionField **input_fields, CeedInt *num_output_fields,
+                                       CeedQFunctionField **output_fields);
+CEED_EXTERN int CeedQFunctionGetNumArgs(CeedQFunction qf, CeedInt *num_input_fields, CeedInt *num_output_fields);
+CEED_EXTERN int CeedQFunctionFieldGetName(CeedQFunctionField qf_field, const char **field_name);
+CEED_EXTERN int CeedQFunctionFieldGetSize(CeedQFunctionField qf_field, CeedInt *size);
+CEED_EXTERN int CeedQFunctionFieldGetEvalMode(CeedQFunctionField qf_field, CeedEvalMode *eval_mode);
+CEED_EXTERN int CeedQFunctionGetSourcePath(CeedQFunction qf, const char **source_path);
+CEED_EXTERN int CeedQFunctionGetKernelName(CeedQFunction qf, const char **kernel_name);
+CEED_EXTERN int CeedQFunctionGetContext(CeedQFunction qf, CeedQFunctionContext *ctx);
+CEED_EXTERN int CeedQFunctionGetData(CeedQFunction qf, void *data);
+CEED_EXTERN int CeedQFunctionSetData(CeedQFunction qf, void *data);
+CEED_EXTERN int CeedQFunctionContextGetCeed(CeedQFunctionContext ctx, Ceed *ceed);
+CEED_EXTERN int CeedQFunctionContextGetState(CeedQFunctionContext ctx, uint64_t *state);
+CEED_EXTERN int CeedQFunctionContextGetBackendData(CeedQFunctionContext ctx, void *data);
+CEED_EXTERN int CeedQFunctionContextSetBackendData(CeedQFunctionContext ctx, void *data);
+
+/* ---- operator ---------------------------------------------------------- */
+CEED_EXTERN int CeedOperatorGetCeed(CeedOperator op, Ceed *ceed);
+CEED_EXTERN int CeedOperatorGetQFunction(CeedOperator op, CeedQFunction *qf);
+CEED_EXTERN int CeedOperatorGetFields(CeedOperator op, CeedInt *num_input_fields, CeedOperatorField **input_fields, CeedInt *num_output_fields,
+                                      CeedOperatorField **output_fields);
+CEED_EXTERN int CeedOperatorFieldGetElemRestriction(CeedOperatorField op_field, CeedElemRestriction *rstr);
+CEED_EXTERN int CeedOperatorFieldGetBasis(CeedOperatorField op_field, CeedBasis *basis);
+CEED_EXTERN int CeedOperatorFieldGetVector(CeedOperatorField op_field, CeedVector *vec);
+CEED_EXTERN int CeedOperatorIsComposite(CeedOperator op, bool *is_composite);
+CEED_EXTERN int CeedOperatorGetData(CeedOperator op, void *data);
+CEED_EXTERN int CeedOperatorSetData(CeedOperator op, void *data);
+
+#endif /* CEED_BACKEND_H */

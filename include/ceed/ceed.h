@@ -1,0 +1,184 @@
+/*
+ * ceed/ceed.h -- the libCEED C API subset provided by this library.
+ *
+ * Function names, signatures, handle types and constants are those of the
+ * reference API (CEED/libCEED include/ceed/ceed.h) so that user code written
+ * for libCEED compiles and runs unchanged; the implementation behind them is
+ * this project's (libceed_amd/host/), with the MI355X backend registered as
+ * the resource "/gpu/hip/mi355".  Only the subset the operator-apply path and
+ * the ex1/ex2/ex3 examples use is provided.
+ */
+#ifndef CEED_H
+#define CEED_H
+
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "types.h"
+
+#ifdef __cplusplus
+#define CEED_EXTERN extern "C" __attribute__((visibility("default")))
+#else
+#define CEED_EXTERN extern __attribute__((visibility("default")))
+#endif
+
+/* ---- opaque handles ---------------------------------------------------- */
+typedef struct Ceed_private                 *Ceed;
+typedef struct CeedRequest_private          *CeedRequest;
+typedef struct CeedVector_private           *CeedVector;
+typedef struct CeedElemRestriction_private  *CeedElemRestriction;
+typedef struct CeedBasis_private            *CeedBasis;
+typedef struct CeedQFunctionField_private   *CeedQFunctionField;
+typedef struct CeedQFunction_private        *CeedQFunction;
+typedef struct CeedOperatorField_private    *CeedOperatorField;
+typedef struct CeedQFunctionContext_private *CeedQFunctionContext;
+typedef struct CeedOperator_private         *CeedOperator;
+
+/* ---- library version --------------------------------------------------- */
+#define CEED_VERSION_MAJOR 0
+#define CEED_VERSION_MINOR 12
+#define CEED_VERSION_PATCH 0
+#define CEED_VERSION_RELEASE false
+CEED_EXTERN int CeedGetVersion(int *major, int *minor, int *patch, bool *release);
+
+/* ---- Ceed context ------------------------------------------------------ */
+CEED_EXTERN int CeedRegistryGetList(size_t *n, char ***const resources, CeedInt **array);
+CEED_EXTERN int CeedInit(const char *resource, Ceed *ceed);
+CEED_EXTERN int CeedReferenceCopy(Ceed ceed, Ceed *ceed_copy);
+CEED_EXTERN int CeedGetResource(Ceed ceed, const char **resource);
+CEED_EXTERN int CeedIsDeterministic(Ceed ceed, bool *is_deterministic);
+CEED_EXTERN int CeedGetPreferredMemType(Ceed ceed, CeedMemType *type);
+CEED_EXTERN int CeedView(Ceed ceed, FILE *stream);
+CEED_EXTERN int CeedDestroy(Ceed *ceed);
+
+/* error handling: CeedError(ceed, code, fmt, ...) routes to the handler of
+ * `ceed` (selected by CEED_ERROR_HANDLER = abort | exit | store | return) */
+CEED_EXTERN int CeedErrorImpl(Ceed ceed, const char *filename, int lineno, const char *func, int ecode, const char *format, ...);
+#define CeedError(ceed, ecode, ...) (CeedErrorImpl((ceed), __FILE__, __LINE__, __func__, (ecode), __VA_ARGS__), (ecode))
+typedef int (*CeedErrorHandler)(Ceed, const char *, int, const char *, int, const char *, va_list *);
+CEED_EXTERN int CeedSetErrorHandler(Ceed ceed, CeedErrorHandler handler);
+CEED_EXTERN int CeedGetErrorMessage(Ceed ceed, const char **err_msg);
+CEED_EXTERN int CeedResetErrorMessage(Ceed ceed, const char **err_msg);
+CEED_EXTERN int CeedErrorReturn(Ceed ceed, const char *filename, int line_no, const char *func, int err_code, const char *format, va_list *args);
+CEED_EXTERN int CeedErrorStore(Ceed ceed, const char *filename, int line_no, const char *func, int err_code, const char *format, va_list *args);
+CEED_EXTERN int CeedErrorAbort(Ceed ceed, const char *filename, int line_no, const char *func, int err_code, const char *format, va_list *args);
+CEED_EXTERN int CeedErrorExit(Ceed ceed, const char *filename, int line_no, const char *func, int err_code, const char *format, va_list *args);
+
+/* ---- special arguments ------------------------------------------------- */
+CEED_EXTERN CeedRequest *const         CEED_REQUEST_IMMEDIATE;
+CEED_EXTERN CeedRequest *const         CEED_REQUEST_ORDERED;
+CEED_EXTERN const CeedVector           CEED_VECTOR_ACTIVE;
+CEED_EXTERN const CeedVector           CEED_VECTOR_NONE;
+CEED_EXTERN const CeedBasis            CEED_BASIS_NONE;
+CEED_EXTERN const CeedElemRestriction  CEED_ELEMRESTRICTION_NONE;
+CEED_EXTERN const CeedQFunction        CEED_QFUNCTION_NONE;
+CEED_EXTERN const CeedInt              CEED_STRIDES_BACKEND[3];
+CEED_EXTERN int                        CeedRequestWait(CeedRequest *req);
+
+/* ---- CeedVector -------------------------------------------------------- */
+CEED_EXTERN int CeedVectorCreate(Ceed ceed, CeedSize len, CeedVector *vec);
+CEED_EXTERN int CeedVectorReferenceCopy(CeedVector vec, CeedVector *vec_copy);
+CEED_EXTERN int CeedVectorCopy(CeedVector vec, CeedVector vec_copy);
+CEED_EXTERN int CeedVectorSetArray(CeedVector vec, CeedMemType mem_type, CeedCopyMode copy_mode, CeedScalar *array);
+CEED_EXTERN int CeedVectorSetValue(CeedVector vec, CeedScalar value);
+CEED_EXTERN int CeedVectorSyncArray(CeedVector vec, CeedMemType mem_type);
+CEED_EXTERN int CeedVectorTakeArray(CeedVector vec, CeedMemType mem_type, CeedScalar **array);
+CEED_EXTERN int CeedVectorGetArray(CeedVector vec, CeedMemType mem_type, CeedScalar **array);
+CEED_EXTERN int CeedVectorGetArrayRead(CeedVector vec, CeedMemType mem_type, const CeedScalar **array);
+CEED_EXTERN int CeedVectorGetArrayWrite(CeedVector vec, CeedMemType mem_type, CeedScalar **array);
+CEED_EXTERN int CeedVectorRestoreArray(CeedVector vec, CeedScalar **array);
+CEED_EXTERN int CeedVectorRestoreArrayRead(CeedVector vec, const CeedScalar **array);
+CEED_EXTERN int CeedVectorNorm(CeedVector vec, CeedNormType type, CeedScalar *norm);
+CEED_EXTERN int CeedVectorScale(CeedVector x, CeedScalar alpha);
+CEED_EXTERN int CeedVectorAXPY(CeedVector y, CeedScalar alpha, CeedVector x);
+CEED_EXTERN int CeedVectorAXPBY(CeedVector y, CeedScalar alpha, CeedScalar beta, CeedVector x);
+CEED_EXTERN int CeedVectorPointwiseMult(CeedVector w, CeedVector x, CeedVector y);
+CEED_EXTERN int CeedVectorReciprocal(CeedVector vec);
+CEED_EXTERN int CeedVectorView(CeedVector vec, const char *fp_fmt, FILE *stream);
+CEED_EXTERN int CeedVectorGetLength(CeedVector vec, CeedSize *length);
+CEED_EXTERN int CeedVectorDestroy(CeedVector *vec);
+
+/* ---- CeedElemRestriction ------------------------------------------------ */
+CEED_EXTERN int CeedElemRestrictionCreate(Ceed ceed, CeedInt num_elem, CeedInt elem_size, CeedInt num_comp, CeedInt comp_stride, CeedSize l_size,
+                                          CeedMemType mem_type, CeedCopyMode copy_mode, const CeedInt *offsets, CeedElemRestriction *rstr);
+CEED_EXTERN int CeedElemRestrictionCreateStrided(Ceed ceed, CeedInt num_elem, CeedInt elem_size, CeedInt num_comp, CeedSize l_size,
+                                                 const CeedInt strides[3], CeedElemRestriction *rstr);
+CEED_EXTERN int CeedElemRestrictionReferenceCopy(CeedElemRestriction rstr, CeedElemRestriction *rstr_copy);
+CEED_EXTERN int CeedElemRestrictionCreateVector(CeedElemRestriction rstr, CeedVector *lvec, CeedVector *evec);
+CEED_EXTERN int CeedElemRestrictionApply(CeedElemRestriction rstr, CeedTransposeMode t_mode, CeedVector u, CeedVector ru, CeedRequest *request);
+CEED_EXTERN int CeedElemRestrictionGetCompStride(CeedElemRestriction rstr, CeedInt *comp_stride);
+CEED_EXTERN int CeedElemRestrictionGetNumElements(CeedElemRestriction rstr, CeedInt *num_elem);
+CEED_EXTERN int CeedElemRestrictionGetElementSize(CeedElemRestriction rstr, CeedInt *elem_size);
+CEED_EXTERN int CeedElemRestrictionGetLVectorSize(CeedElemRestriction rstr, CeedSize *l_size);
+CEED_EXTERN int CeedElemRestrictionGetEVectorSize(CeedElemRestriction rstr, CeedSize *e_size);
+CEED_EXTERN int CeedElemRestrictionGetNumComponents(CeedElemRestriction rstr, CeedInt *num_comp);
+CEED_EXTERN int CeedElemRestrictionGetMultiplicity(CeedElemRestriction rstr, CeedVector mult);
+CEED_EXTERN int CeedElemRestrictionDestroy(CeedElemRestriction *rstr);
+
+/* ---- CeedBasis ---------------------------------------------------------- */
+CEED_EXTERN int CeedBasisCreateTensorH1Lagrange(Ceed ceed, CeedInt dim, CeedInt num_comp, CeedInt P, CeedInt Q, CeedQuadMode quad_mode,
+                                                CeedBasis *basis);
+CEED_EXTERN int CeedBasisCreateTensorH1(Ceed ceed, CeedInt dim, CeedInt num_comp, CeedInt P_1d, CeedInt Q_1d, const CeedScalar *interp_1d,
+                                        const CeedScalar *grad_1d, const CeedScalar *q_ref_1d, const CeedScalar *q_weight_1d, CeedBasis *basis);
+CEED_EXTERN int CeedBasisReferenceCopy(CeedBasis basis, CeedBasis *basis_copy);
+CEED_EXTERN int CeedBasisView(CeedBasis basis, FILE *stream);
+CEED_EXTERN int CeedBasisApply(CeedBasis basis, CeedInt num_elem, CeedTransposeMode t_mode, CeedEvalMode eval_mode, CeedVector u, CeedVector v);
+CEED_EXTERN int CeedBasisApplyAdd(CeedBasis basis, CeedInt num_elem, CeedTransposeMode t_mode, CeedEvalMode eval_mode, CeedVector u, CeedVector v);
+CEED_EXTERN int CeedBasisGetDimension(CeedBasis basis, CeedInt *dim);
+CEED_EXTERN int CeedBasisGetNumComponents(CeedBasis basis, CeedInt *num_comp);
+CEED_EXTERN int CeedBasisGetNumNodes(CeedBasis basis, CeedInt *P);
+CEED_EXTERN int CeedBasisGetNumNodes1D(CeedBasis basis, CeedInt *P_1d);
+CEED_EXTERN int CeedBasisGetNumQuadraturePoints(CeedBasis basis, CeedInt *Q);
+CEED_EXTERN int CeedBasisGetNumQuadraturePoints1D(CeedBasis basis, CeedInt *Q_1d);
+CEED_EXTERN int CeedBasisGetQRef(CeedBasis basis, const CeedScalar **q_ref);
+CEED_EXTERN int CeedBasisGetQWeights(CeedBasis basis, const CeedScalar **q_weights);
+CEED_EXTERN int CeedBasisGetInterp1D(CeedBasis basis, const CeedScalar **interp_1d);
+CEED_EXTERN int CeedBasisGetGrad1D(CeedBasis basis, const CeedScalar **grad_1d);
+CEED_EXTERN int CeedBasisDestroy(CeedBasis *basis);
+CEED_EXTERN int CeedGaussQuadrature(CeedInt Q, CeedScalar *q_ref_1d, CeedScalar *q_weight_1d);
+CEED_EXTERN int CeedLobattoQuadrature(CeedInt Q, CeedScalar *q_ref_1d, CeedScalar *q_weight_1d);
+
+/* ---- CeedQFunction ------------------------------------------------------ */
+typedef int (*CeedQFunctionUser)(void *ctx, const CeedInt Q, const CeedScalar *const *in, CeedScalar *const *out);
+CEED_EXTERN int CeedQFunctionCreateInterior(Ceed ceed, CeedInt vec_length, CeedQFunctionUser f, const char *source, CeedQFunction *qf);
+CEED_EXTERN int CeedQFunctionCreateInteriorByName(Ceed ceed, const char *name, CeedQFunction *qf);
+CEED_EXTERN int CeedQFunctionReferenceCopy(CeedQFunction qf, CeedQFunction *qf_copy);
+CEED_EXTERN int CeedQFunctionAddInput(CeedQFunction qf, const char *field_name, CeedInt size, CeedEvalMode eval_mode);
+CEED_EXTERN int CeedQFunctionAddOutput(CeedQFunction qf, const char *field_name, CeedInt size, CeedEvalMode eval_mode);
+CEED_EXTERN int CeedQFunctionSetContext(CeedQFunction qf, CeedQFunctionContext ctx);
+CEED_EXTERN int CeedQFunctionSetUserFlopsEstimate(CeedQFunction qf, CeedSize flops);
+CEED_EXTERN int CeedQFunctionApply(CeedQFunction qf, CeedInt Q, CeedVector *u, CeedVector *v);
+CEED_EXTERN int CeedQFunctionDestroy(CeedQFunction *qf);
+
+/* ---- CeedQFunctionContext ---------------------------------------------- */
+CEED_EXTERN int CeedQFunctionContextCreate(Ceed ceed, CeedQFunctionContext *ctx);
+CEED_EXTERN int CeedQFunctionContextReferenceCopy(CeedQFunctionContext ctx, CeedQFunctionContext *ctx_copy);
+CEED_EXTERN int CeedQFunctionContextSetData(CeedQFunctionContext ctx, CeedMemType mem_type, CeedCopyMode copy_mode, size_t size, void *data);
+CEED_EXTERN int CeedQFunctionContextGetData(CeedQFunctionContext ctx, CeedMemType mem_type, void *data);
+CEED_EXTERN int CeedQFunctionContextGetDataRead(CeedQFunctionContext ctx, CeedMemType mem_type, void *data);
+CEED_EXTERN int CeedQFunctionContextRestoreData(CeedQFunctionContext ctx, void *data);
+CEED_EXTERN int CeedQFunctionContextRestoreDataRead(CeedQFunctionContext ctx, void *data);
+CEED_EXTERN int CeedQFunctionContextGetContextSize(CeedQFunctionContext ctx, size_t *ctx_size);
+CEED_EXTERN int CeedQFunctionContextDestroy(CeedQFunctionContext *ctx);
+
+/* ---- CeedOperator ------------------------------------------------------- */
+CEED_EXTERN int CeedOperatorCreate(Ceed ceed, CeedQFunction qf, CeedQFunction dqf, CeedQFunction dqfT, CeedOperator *op);
+CEED_EXTERN int CeedOperatorReferenceCopy(CeedOperator op, CeedOperator *op_copy);
+CEED_EXTERN int CeedOperatorSetField(CeedOperator op, const char *field_name, CeedElemRestriction rstr, CeedBasis basis, CeedVector vec);
+CEED_EXTERN int CeedOperatorCheckReady(CeedOperator op);
+CEED_EXTERN int CeedOperatorGetNumElements(CeedOperator op, CeedInt *num_elem);
+CEED_EXTERN int CeedOperatorGetNumQuadraturePoints(CeedOperator op, CeedInt *num_qpts);
+CEED_EXTERN int CeedOperatorApply(CeedOperator op, CeedVector in, CeedVector out, CeedRequest *request);
+CEED_EXTERN int CeedOperatorApplyAdd(CeedOperator op, CeedVector in, CeedVector out, CeedRequest *request);
+CEED_EXTERN int CeedOperatorDestroy(CeedOperator *op);
+
+/* ---- utilities ---------------------------------------------------------- */
+static inline CeedInt CeedIntPow(CeedInt base, CeedInt power) {
+  CeedInt r = 1;
+  for (CeedInt i = 0; i < power; i++) r *= base;
+  return r;
+}
+static inline CeedInt CeedIntMin(CeedInt a, CeedInt b) { return a < b ? a : b; }
+static inline CeedInt CeedIntMax(CeedInt a, CeedInt b) { return a > b ? a : b; }
+
+#endif /* CEED_H */

@@ -59,3 +59,43 @@ def test_host_quadrature_matches_oracle():
 def test_error_reporting():
     with pytest.raises(_lib.CeedMi355Error, match="CeedGaussQuadrature"):
         ops.gauss_quadrature(0)
+
+
+# ------------------------------------------------------------ libCEED API --
+REFERENCE_INCLUDE = Path("/root/reference/include")
+HOST = Path(__file__).resolve().parent.parent / "libceed_amd" / "host"
+
+
+def test_registry_lists_backend():
+    from libceed_amd import ceed as libceed
+
+    assert "/gpu/hip/mi355" in libceed.registry()
+
+
+def test_init_without_gpu_fails_loudly():
+    """No CPU fallback: on a machine without a GPU, CeedInit of the backend is an error."""
+    import torch
+
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    from libceed_amd import ceed as libceed
+
+    with pytest.raises(libceed.CeedError):
+        libceed.Ceed("/gpu/hip/mi355")
+    with pytest.raises(libceed.CeedError):
+        libceed.Ceed("/cpu/self/ref/serial")  # not provided by this library
+
+
+@pytest.mark.skipif(not REFERENCE_INCLUDE.exists(), reason="reference headers not present")
+@pytest.mark.parametrize("src", ["ceed-mi355-objects.c", "ceed-mi355-operator.c"])
+def test_backend_compiles_against_reference_headers(tmp_path, src):
+    """The backend is written to the reference's backend API: it compiles
+    unchanged against the reference's include/ceed/{ceed,backend}.h."""
+    import shutil
+    import subprocess
+
+    for h in ("ceed-mi355.h", "ceed-mi355-kernels.h"):
+        shutil.copy(INCLUDE / h, tmp_path / h)
+    res = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-fsyntax-only", f"-I{REFERENCE_INCLUDE}", f"-I{tmp_path}", f"-I{HOST}",
+                          str(HOST / src)], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
