@@ -1,7 +1,8 @@
 """Build the MI355X (gfx950) kernel library in-tree.
 
 Compiles every ``csrc/*.hip`` with hipcc for gfx950 into objects under
-``_build/`` and links ``libceed_amd/lib/libceed-mi355.so``.  The host-side C
+``_build/`` and links ``libceed_amd/lib/libceed-mi355.so``, plus ``libceed-mi355-kernels.so`` (the
+kernel ABI alone, for linking the backend into a libCEED build).  The host-side C
 layer (``host/*.c``, the libCEED-API core and the /gpu/hip/mi355 backend) is
 linked into the same library when present.
 
@@ -22,6 +23,7 @@ ROOT = PKG.parent
 BUILD = ROOT / "_build"
 LIB_DIR = PKG / "lib"
 LIB = LIB_DIR / "libceed-mi355.so"
+KERNELS_LIB = LIB_DIR / "libceed-mi355-kernels.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -80,10 +82,15 @@ def build(jobs: int | None = None, force: bool = True, verbose: bool = False) ->
             if verbose:
                 print(" ".join(cmd[:1] + cmd[-3:]), flush=True)
             fut.result()
+    link = ["-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"]
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
-          "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), *link])
     os.replace(tmp, LIB)
+    # kernels only (no libCEED API core): what a libCEED build links the backend against
+    kobjs = [o for o in objs if not o.name.startswith("host_")]
+    tmp = KERNELS_LIB.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, kobjs), *link])
+    os.replace(tmp, KERNELS_LIB)
     return LIB
 
 
