@@ -7,12 +7,14 @@ GDoF/s = 1e-9 * DoFs * applies / second.  One step = one CeedOperatorApply
 (zero v, then the fused E^T B^T D B E kernels over all element colours), with
 inputs resident in HBM.
 
-Also reported: a sweep over p = 1..8 at the same size in FP64 and FP32, the
-roofline of the fused kernel (algorithmic bytes per apply / kernel time,
+Also reported: a sweep over p = 1..8 at the same size in FP64 and FP32; CG
+(200 FP64 iterations at p = 4, 8M DoFs; and time to a 1e-10 relative
+residual at 16M DoFs, p = 4 and 6, FP64 CG against FP32 CG with the residual
+replaced by the FP64 one every 25 iterations); the roofline of the fused kernel (algorithmic bytes per apply / kernel time,
 against 8 TB/s), and a CPU baseline: the reference's /cpu/self/avx/blocked
 (built from its sources, oracle/_ref) timed on this host, one process per core.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--p 4] [--dofs 8e6] [--no-sweep] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--p 4] [--dofs 8e6] [--no-sweep] [--no-cpu] [--no-cg]
 """
 
 from __future__ import annotations
@@ -92,6 +94,75 @@ def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0)
     return res
 
 
+def _cg_problem(p: int, dofs: float, seed: int = 0):
+    """Diffusion operator (FP64 and FP32 plans) and rhs = A u_exact, u_exact = sin(pi x) sin(pi y) sin(pi z) at the nodes."""
+    import torch
+
+    from libceed_amd import ops
+    from libceed_amd.mesh import hex_mesh, mesh_size_for_dofs, node_coordinates
+
+    m = hex_mesh(mesh_size_for_dofs(p, dofs), p, seed=seed)
+    Q = p + 2
+    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q)
+    gll = np.array(ops.lobatto_quadrature(p + 1)[0])
+    xyz = node_coordinates(m, gll)
+    u = torch.from_numpy(np.sin(np.pi * xyz[0]) * np.sin(np.pi * xyz[1]) * np.sin(np.pi * xyz[2])).cuda()
+    b = torch.zeros_like(u)
+    op.apply(u, b)
+    return m, Q, op, b
+
+
+def run_cg_fixed(p: int, dofs: float, iterations: int = 200) -> dict:
+    """FP64 CG, a fixed number of iterations (the benchmark problems' "DoFs/s in CG")."""
+    import torch
+
+    from libceed_amd import ops
+
+    m, Q, op, b = _cg_problem(p, dofs)
+    cg = ops.CG(op)
+    x = torch.zeros_like(b)
+    cg.solve(b, x, max_iter=10, rtol=0.0, check_every=10)  # warm-up: graph capture
+    x.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = cg.solve(b, x, max_iter=iterations, rtol=0.0, check_every=50)
+    sec = time.perf_counter() - t0
+    return {"p": p, "dofs": m.num_nodes, "iterations": st.iterations, "seconds": round(sec, 4),
+            "ms_per_iteration": round(1e3 * sec / st.iterations, 4), "gdofs_in_cg": round(m.num_nodes * st.iterations / sec / 1e9, 3),
+            "rel_residual": st.rel_residual}
+
+
+def run_cg_mixed(p: int, dofs: float, k: int = 25, rtol: float = 1e-10) -> dict:
+    """Time to 1e-10: FP64 CG against FP32 CG with FP64 residual replacement every k iterations."""
+    import torch
+
+    from libceed_amd import ops
+
+    m, Q, op, b = _cg_problem(p, dofs)
+    op32 = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q, dtype=torch.float32)
+    out = {"p": p, "dofs": m.num_nodes, "k": k, "rtol": rtol}
+    xs = {}
+    for name, solver, every in (("fp64", ops.CG(op), 50), ("fp32_refine", ops.CG(op32, op), k)):
+        x = torch.zeros_like(b)
+        solver.solve(b, x, max_iter=2 * every, rtol=0.0, check_every=every)  # warm-up: graph capture
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = solver.solve(b, x, max_iter=20000, rtol=rtol, check_every=every)
+        sec = time.perf_counter() - t0
+        xs[name] = x
+        out[name] = {"seconds": round(sec, 4), "iterations": st.iterations, "rel_residual": st.rel_residual,
+                     "ms_per_iteration": round(1e3 * sec / max(1, st.iterations), 4)}
+        if name == "fp32_refine":
+            out[name]["residual_replacements"] = st.replacements
+    d = xs["fp32_refine"] - xs["fp64"]
+    x64 = xs["fp64"]
+    # the operator is pure Neumann: compare up to constants
+    out["fp32_refine"]["rel_error_vs_fp64"] = (torch.linalg.norm(d - d.mean()) / torch.linalg.norm(x64 - x64.mean())).item()
+    out["speedup"] = round(out["fp64"]["seconds"] / out["fp32_refine"]["seconds"], 3)
+    return out
+
+
 def cpu_baseline(p: int = 4, seconds: float = 10.0) -> dict | None:
     """Reference /cpu/self/avx/blocked on this host's cores (one single-threaded process per core)."""
     exe = ROOT / "oracle" / "_ref" / "ref_cpu_bench"
@@ -158,6 +229,7 @@ def main() -> None:
     ap.add_argument("--dofs", type=float, default=8e6)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 1 GiB device copy (HBM reference for bandwidth and counter scaling)")
     args = ap.parse_args()
@@ -187,6 +259,14 @@ def main() -> None:
             sweep[str(p)] = {"dofs": r64["dofs"], "fp64_gdofs": round(r64["gdofs"], 3), "fp64_hbm_frac": round(r64["hbm_frac"], 4),
                              "fp32_gdofs": round(r32["gdofs"], 3), "fp32_hbm_frac": round(r32["hbm_frac"], 4)}
             print(f"# p={p}: {sweep[str(p)]}", file=sys.stderr, flush=True)
+    cg = {}
+    if not args.no_cg:
+        cg["fp64_200_iterations_p4_8M"] = run_cg_fixed(4, 8e6)
+        print(f"# cg: {cg['fp64_200_iterations_p4_8M']}", file=sys.stderr, flush=True)
+        for p in (4, 6):
+            cg[f"to_1e-10_p{p}_16M"] = run_cg_mixed(p, 16e6)
+            print(f"# cg p={p}: {cg[f'to_1e-10_p{p}_16M']}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
     cpu = None if args.no_cpu else cpu_baseline(args.p)
     line = {
         "metric": "3D diffusion apply GDoF/s p=1..8 (FP64 vs FP32+refine @1e-10), % HBM peak",
@@ -211,6 +291,7 @@ def main() -> None:
                      "fp64_tflops": round(head["kernel_TFLOPs"], 2), "fp64_peak_tflops": FP64_PEAK_TFLOPS},
         "cpu_baseline": cpu,
         "sweep": sweep,
+        "cg": cg,
     }
     print(json.dumps(line), flush=True)
 

@@ -182,6 +182,34 @@ int CeedMi355QFunctionApply(CeedMi355QFunction qf, int64_t num_elem, int32_t Q, 
                             double *const *out_host, void *stream);
 int CeedMi355QFunctionDestroy(CeedMi355QFunction *qf);
 
+/* ------------------------------------------------------------------------ */
+/* Conjugate gradients on a fused plan                                       */
+/* ------------------------------------------------------------------------ */
+typedef struct CeedMi355CG_private *CeedMi355CG;
+typedef struct {
+  int32_t iterations;             /* inner iterations run */
+  int32_t replacements;           /* FP64 residual replacements (mixed precision) */
+  double  rel_residual;           /* ||b - A x|| / ||b||, recomputed in FP64 at the end */
+  double  rel_residual_recursive; /* the value the stopping test last saw */
+} CeedMi355CGStats;
+
+/* Unpreconditioned CG with device-resident scalars.  op64 == NULL: plain FP64
+ * CG on the FP64 plan `op`.  op64 != NULL: `op` is an FP32 plan of the same
+ * operator; the iteration runs in FP32 and every check_every iterations the
+ * FP32 increment is added to the FP64 solution and the residual is replaced
+ * by b - A64 x computed with op64.  The solver the reference's benchmark
+ * problems run around CeedOperatorApply (PETSc KSPCG, examples/petsc/bps.c:210-239). */
+int CeedMi355CGCreate(CeedMi355Operator op, CeedMi355Operator op64, CeedMi355CG *cg);
+/* Solve A x = b from the given x (FP64 L-vectors).  Stops when
+ * ||r|| <= rtol ||b|| (tested every check_every iterations; the recursive
+ * residual for plain CG, the replaced FP64 residual for mixed precision) or
+ * after max_iter iterations.  use_graph != 0 captures check_every iterations
+ * in a hipGraph once and replays it.  Ordered after the caller's stream;
+ * returns when the solve is done. */
+int CeedMi355CGSolve(CeedMi355CG cg, const double *b, double *x, int32_t max_iter, double rtol, int32_t check_every, int use_graph, void *stream,
+                     CeedMi355CGStats *stats_host);
+int CeedMi355CGDestroy(CeedMi355CG *cg);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,9 @@ SIGNATURES = {
     "CeedMi355QFunctionCompile": (c_int, [C.c_char_p, C.c_char_p, c_i32, vp, c_i32, vp, C.POINTER(vp)]),
     "CeedMi355QFunctionApply": (c_int, [vp, c_i64, c_i32, vp, vp, vp, vp]),
     "CeedMi355QFunctionDestroy": (c_int, [C.POINTER(vp)]),
+    "CeedMi355CGCreate": (c_int, [vp, vp, C.POINTER(vp)]),
+    "CeedMi355CGSolve": (c_int, [vp, vp, vp, c_i32, C.c_double, c_i32, c_int, vp, vp]),
+    "CeedMi355CGDestroy": (c_int, [C.POINTER(vp)]),
     "CeedRegister_Mi355": (c_int, []),
     "CeedInit_Mi355": (c_int, [C.c_char_p, vp]),
 }

@@ -106,7 +106,7 @@ class Restriction:
         call("CeedMi355RestrictionApply", self._h, t_mode, _ptr(u), _ptr(v), _stream())
 
     def __del__(self):
-        if getattr(self, "_h", None) and _lib._lib is not None:
+        if getattr(self, "_h", None) and _lib is not None and _lib._lib is not None:
             _lib.lib().CeedMi355RestrictionDestroy(C.byref(self._h))
 
 
@@ -131,7 +131,7 @@ class Basis:
         call("CeedMi355BasisApply", self._h, num_elem, t_mode, eval_mode, int(add), _ptr(u), _ptr(v), _stream())
 
     def __del__(self):
-        if getattr(self, "_h", None) and _lib._lib is not None:
+        if getattr(self, "_h", None) and _lib is not None and _lib._lib is not None:
             _lib.lib().CeedMi355BasisDestroy(C.byref(self._h))
 
 
@@ -210,5 +210,37 @@ class HexOperator:
                 raise ValueError(f"L-vector has {a.numel()} entries, operator needs {self.l_size}")
 
     def __del__(self):
-        if getattr(self, "_h", None) and _lib._lib is not None:
+        if getattr(self, "_h", None) and _lib is not None and _lib._lib is not None:
             _lib.lib().CeedMi355OperatorDestroy(C.byref(self._h))
+
+
+class CGStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("replacements", C.c_int32), ("rel_residual", C.c_double),
+                ("rel_residual_recursive", C.c_double)]
+
+
+class CG:
+    """Conjugate gradients on the GPU (CeedMi355CGCreate/Solve).
+
+    CG(op64): plain FP64 CG.  CG(op32, op64): FP32 iteration with the residual
+    replaced by the FP64 one every ``check_every`` iterations.
+    """
+
+    def __init__(self, op: HexOperator, op64: HexOperator | None = None):
+        self.op, self.op64 = op, op64
+        h = C.c_void_p()
+        call("CeedMi355CGCreate", op._h, op64._h if op64 is not None else None, C.byref(h))
+        self._h = h
+
+    def solve(self, b: torch.Tensor, x: torch.Tensor, max_iter: int = 1000, rtol: float = 1e-10, check_every: int = 25,
+              use_graph: bool = True) -> CGStats:
+        for a in (b, x):
+            _device_tensor(a, torch.float64)
+        st = CGStats()
+        call("CeedMi355CGSolve", self._h, _ptr(b), _ptr(x), int(max_iter), float(rtol), int(check_every), int(use_graph), _stream(),
+             C.byref(st))
+        return st
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None and _lib._lib is not None:
+            _lib.lib().CeedMi355CGDestroy(C.byref(self._h))

@@ -1,0 +1,111 @@
+"""Conjugate gradients on the GPU (plain FP64, and FP32 iteration with FP64
+residual replacement) on the singular pure-Neumann diffusion operator.
+
+Checks: the solution equals the minimum-norm solution pinv(A) b computed from
+the dense operator the oracle assembles (CG started from 0 with b in range(A)
+stays in range(A)); the true FP64 residual meets the tolerance; graph replay
+gives the same bits as the eager iteration; a fixed iteration budget is
+honoured.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from libceed_amd import ops
+from libceed_amd.mesh import hex_mesh, node_coordinates
+
+pytestmark = pytest.mark.gpu
+
+
+def dense_operator(m, Q, qd):
+    N = m.num_nodes
+    A = np.zeros((N, N))
+    for j in range(N):
+        e = np.zeros(N)
+        e[j] = 1.0
+        A[:, j] = oracle.hex_apply(1, 1, m.num_elem, m.P, Q, m.offsets, N, qd, e)
+    return A
+
+
+def exact_rhs(m, op):
+    xyz = node_coordinates(m, oracle.lobatto(m.P)[0])
+    u = np.sin(np.pi * xyz[0]) * np.sin(np.pi * xyz[1]) * np.sin(np.pi * xyz[2])
+    ud = torch.from_numpy(u).cuda()
+    b = torch.zeros_like(ud)
+    op.apply(ud, b)
+    return u, b
+
+
+@pytest.fixture(scope="module")
+def small():
+    m = hex_mesh(3, 2, seed=13)
+    Q = 4
+    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q)
+    qd = oracle.hex_build_qdata(1, m, Q)
+    return m, Q, op, qd
+
+
+def test_cg_min_norm_solution(cuda, small):
+    m, Q, op, qd = small
+    u, b = exact_rhs(m, op)
+    x = torch.zeros_like(b)
+    st = ops.CG(op).solve(b, x, max_iter=2000, rtol=1e-12, check_every=5)
+    assert st.rel_residual < 1e-11, st.rel_residual
+    A = dense_operator(m, Q, qd)
+    x_ref = np.linalg.pinv(A, rcond=1e-12) @ b.cpu().numpy()
+    xh = x.cpu().numpy()
+    assert np.linalg.norm(xh - x_ref) / np.linalg.norm(x_ref) < 1e-9
+    # the exact solution differs from the min-norm one by a constant
+    d = xh - u
+    assert np.abs(d - d.mean()).max() < 1e-8  # b = A u exactly, so x = u + constant
+
+
+def test_cg_mixed_precision(cuda, small):
+    m, Q, op, _ = small
+    _, b = exact_rhs(m, op)
+    op32 = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q, dtype=torch.float32)
+    x64 = torch.zeros_like(b)
+    ops.CG(op).solve(b, x64, max_iter=2000, rtol=1e-11, check_every=5)
+    for k in (10, 25):
+        x = torch.zeros_like(b)
+        st = ops.CG(op32, op).solve(b, x, max_iter=4000, rtol=1e-10, check_every=k)
+        assert st.rel_residual <= 1e-10, (k, st.rel_residual)
+        assert st.replacements >= 1
+        # equal up to the null space of the pure-Neumann operator (constants)
+        d = x - x64
+        assert (torch.linalg.norm(d - d.mean()) / torch.linalg.norm(x64)).item() < 1e-9
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_cg_graph_matches_eager(cuda, mixed):
+    m = hex_mesh(6, 3, seed=2)
+    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=5)
+    solver = ops.CG(ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=5, dtype=torch.float32), op) if mixed else ops.CG(op)
+    _, b = exact_rhs(m, op)
+    xs, stats = [], []
+    for use_graph in (False, True, True, False, True):
+        x = torch.zeros_like(b)
+        stats.append(solver.solve(b, x, max_iter=150, rtol=1e-14, check_every=10, use_graph=use_graph))
+        torch.cuda.synchronize()
+        xs.append(x.cpu().numpy())
+    for x, st in zip(xs[1:], stats[1:]):
+        assert np.array_equal(x, xs[0])
+        assert st.iterations == stats[0].iterations == 150
+        assert st.rel_residual == stats[0].rel_residual
+
+
+def test_cg_fixed_iterations(cuda):
+    """200 iterations as a fixed budget (check_every need not divide it)."""
+    m = hex_mesh(8, 4, seed=5)
+    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=6)
+    _, b = exact_rhs(m, op)
+    x = torch.zeros_like(b)
+    st = ops.CG(op).solve(b, x, max_iter=200, rtol=0.0, check_every=30)
+    assert st.iterations == 200
+    assert 0 < st.rel_residual < 1e-2
+    # residual decreases with more iterations
+    x2 = torch.zeros_like(b)
+    st2 = ops.CG(op).solve(b, x2, max_iter=400, rtol=0.0, check_every=30)
+    assert st2.rel_residual < st.rel_residual
