@@ -45,6 +45,9 @@ enum { CEED_MI355_FP64 = 0, CEED_MI355_FP32 = 1 };
  * kappa-weighted Poisson3DApply for -div(kappa grad u)). */
 enum { CEED_MI355_QF_MASS = 0, CEED_MI355_QF_DIFFUSION = 1, CEED_MI355_QF_DIFFUSION_KAPPA = 2 };
 
+/* Plan flags of CeedMi355OperatorCreateHex. */
+enum { CEED_MI355_PLAN_DETERMINISTIC = 1 };
+
 /* Geometric-factor builders (gallery Mass3DBuild / Poisson3DBuild). */
 enum { CEED_MI355_BUILD_MASS = 0, CEED_MI355_BUILD_DIFFUSION = 1 };
 
@@ -103,15 +106,18 @@ int CeedMi355BasisDestroy(CeedMi355Basis *basis);
 /* ------------------------------------------------------------------------ */
 typedef struct CeedMi355Operator_private *CeedMi355Operator;
 
-/* Plan for a 3D tensor hex operator with a gallery pointwise D.  Colours the
- * elements so that no two elements of one colour share an L-vector entry,
- * and stores offsets and geometric factors in colour order, so the fused
- * kernel's E^T needs no atomics.  Supports Q_1d >= P_1d, P_1d in 2..10.
+/* Plan for a 3D tensor hex operator with a gallery pointwise D, run as one
+ * fused kernel per apply.  E^T: by default hardware atomic adds (one launch
+ * over all elements in element order; sums in arrival order).  With
+ * CEED_MI355_PLAN_DETERMINISTIC the elements are coloured so that no two
+ * elements of a colour share an L-vector entry and stored in colour order;
+ * one launch per colour then adds without atomics, and results are bitwise
+ * reproducible.  Supports Q_1d in {P_1d, P_1d + 1}, P_1d in 2..9.
  * scalar_type selects FP64 or FP32 storage and arithmetic for u, v and
  * q_data.  Replaces the per-operator set-up of ceed-ref-operator.c:145-210. */
 int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_comp, int32_t num_elem, int32_t P_1d, int32_t Q_1d,
                                const double *interp_1d_host, const double *grad_1d_host, const int32_t *offsets_host, int32_t comp_stride,
-                               int64_t l_size, CeedMi355Operator *op);
+                               int64_t l_size, int flags, CeedMi355Operator *op);
 /* Geometric factors from mesh coordinates on the GPU, FP64 arithmetic:
  * Mass3DBuild (w det J) or Poisson3DBuild (6 Voigt entries of
  * w det J J^-1 J^-T).  The mesh restriction has mesh_P^3 nodes per element,

@@ -20,6 +20,9 @@
 namespace mi355 {
 
 enum { kQfMass = 0, kQfDiffusion = 1, kQfDiffusionKappa = 2 };
+// E^T modes: colour-exclusive read-modify-write (ApplyAdd), the same with the
+// first-touch overwrite (Apply), or hardware atomics (one launch, any order)
+enum { kScatterAdd = 0, kScatterOverwrite = 1, kScatterAtomic = 2 };
 
 template <int QF>
 struct QfTraits;
@@ -69,7 +72,7 @@ __device__ __forceinline__ int64_t xcd_chunk(int64_t b, int64_t grid) {
 }
 
 template <int P, int Q, int NCOMP, int QF, typename T>
-__global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int64_t elem_end, int overwrite,
+__global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int64_t elem_end, int scatter,
                                                          const int32_t *__restrict__ offsets, int32_t comp_stride,
                                                          const T *__restrict__ q_data, const T *__restrict__ mats, const T *__restrict__ u,
                                                          T *__restrict__ v) {
@@ -233,7 +236,8 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
 #pragma unroll
         for (int a = 0; a < Q; a++) s += sB[a * P + tx] * S1[sidx(z, ty, a)];
         T *dst = &v[(off[z] >> 1) + coff];
-        if (overwrite && (off[z] & 1)) *dst = s;  // first element to reach this entry: CeedOperatorApply writes
+        if (scatter == kScatterAtomic) unsafeAtomicAdd(dst, s);                        // all elements in one launch
+        else if (scatter == kScatterOverwrite && (off[z] & 1)) *dst = s;  // first element to reach this entry: CeedOperatorApply writes
         else *dst += s;
       }
     }

@@ -30,6 +30,7 @@ struct CeedMi355Operator_private {
   int64_t l_size;
   int     epb;
   bool    covers_all = false;  // every L entry is reached by some element: Apply needs no zero fill
+  bool    atomic     = false;  // one colour, E^T by hardware atomics (not bitwise reproducible)
   // colour c owns packed elements [color_start[c], color_start[c+1])
   std::vector<int64_t> color_start;
   std::vector<int32_t> perm;  // packed position -> caller's element index
@@ -51,7 +52,8 @@ int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v
     const int64_t b = op->color_start[c], e = op->color_start[c + 1];
     if (e <= b) continue;
     const int64_t grid = (e - b + epb - 1) / epb;
-    hex_apply_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, overwrite, op->d_offsets, op->comp_stride, (const T *)op->d_qdata,
+    const int scatter = op->atomic ? kScatterAtomic : (overwrite ? kScatterOverwrite : kScatterAdd);
+    hex_apply_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, scatter, op->d_offsets, op->comp_stride, (const T *)op->d_qdata,
                                                                           (const T *)op->d_mats, (const T *)u, (T *)v);
   }
   MI355_LAUNCH_CHECK();

@@ -212,7 +212,7 @@ int64_t elem_block(const CeedMi355Operator op) {
 
 extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_comp, int32_t num_elem, int32_t P_1d, int32_t Q_1d,
                                           const double *interp_1d_host, const double *grad_1d_host, const int32_t *offsets_host,
-                                          int32_t comp_stride, int64_t l_size, CeedMi355Operator *op_out) {
+                                          int32_t comp_stride, int64_t l_size, int flags, CeedMi355Operator *op_out) {
   *op_out = nullptr;
   MI355_CHECK(qf == kQfMass || qf == kQfDiffusion || qf == kQfDiffusionKappa, CEED_MI355_ERROR_UNSUPPORTED, "unknown pointwise operator %d", qf);
   MI355_CHECK(scalar_type == CEED_MI355_FP64 || scalar_type == CEED_MI355_FP32, CEED_MI355_ERROR_UNSUPPORTED, "unknown scalar type");
@@ -242,7 +242,12 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
   const int            es = P_1d * P_1d * P_1d;
   std::vector<int32_t> color;
   int                  num_colors = 0;
-  int ierr = color_elements(offsets_host, num_elem, es, num_comp, comp_stride, l_size, color, num_colors);
+  op->atomic = !(flags & CEED_MI355_PLAN_DETERMINISTIC);
+  int ierr = op->atomic ? CEED_MI355_SUCCESS : color_elements(offsets_host, num_elem, es, num_comp, comp_stride, l_size, color, num_colors);
+  if (op->atomic) {  // one colour in element order
+    color.assign((size_t)num_elem, 0);
+    num_colors = num_elem > 0 ? 1 : 0;
+  }
   if (ierr) {
     delete op;
     return ierr;
@@ -270,7 +275,7 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
     for (int64_t p = 0; p < num_elem; p++)
       for (int i = 0; i < es; i++) {
         const int32_t node = offsets_host[(int64_t)op->perm[p] * es + i];
-        packed[p * es + i] = 2 * node + (touched[node] ? 0 : 1);
+        packed[p * es + i] = 2 * node + (touched[node] || op->atomic ? 0 : 1);
         touched[node]      = 1;
       }
     std::vector<uint8_t> hit(l_size, 0);
@@ -284,7 +289,7 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
           }
           h = 1;
         }
-    op->covers_all = std::all_of(hit.begin(), hit.end(), [](uint8_t h) { return h != 0; });
+    op->covers_all = !op->atomic && std::all_of(hit.begin(), hit.end(), [](uint8_t h) { return h != 0; });
   }
 
   // B and the collocated derivative D = G pinv(B), in the plan's scalar type

@@ -4,8 +4,9 @@
  * Fused path: an operator made of one gallery pointwise operator
  * (MassApply, Vector3MassApply, Poisson3DApply, Vector3Poisson3DApply), an
  * active field with a standard restriction and a 3D tensor basis used for both
- * input and output, and passive geometric factors (EVAL_NONE), runs as ONE
- * kernel per element colour: v (+)= E^T B^T D B E u with E^T atomic-free (see
+ * input and output, and passive geometric factors (EVAL_NONE), runs as one
+ * fused kernel: v (+)= E^T B^T D B E u, with E^T by atomic adds, or, on a
+ * deterministic resource, by one atomic-free launch per element colour (see
  * the CeedMi355OperatorCreateHex plan).  This is the path of the reference's
  * fused GPU backends (backends/hip-gen/ceed-hip-gen-operator.c:52-258),
  * rebuilt for gfx950.
@@ -135,7 +136,10 @@ static int try_fused(CeedOperator op, OperatorData *d) {
     CeedCallBackend(CeedElemRestrictionGetOffsets(ru, CEED_MEM_HOST, &offsets));
     /* shapes the fused kernels are instantiated for; anything else takes the generic path */
     if (elem_size == P * P * P && P >= 2 && P <= 9 && (Q == P || Q == P + 1)) {
-      int ierr = CeedMi355OperatorCreateHex(kind, CEED_MI355_FP64, num_comp, num_elem, P, Q, interp, grad, offsets, comp_stride, d->l_size, &d->hex);
+      bool deterministic;
+      CeedCallBackend(CeedIsDeterministic(ceed, &deterministic));
+      int ierr = CeedMi355OperatorCreateHex(kind, CEED_MI355_FP64, num_comp, num_elem, P, Q, interp, grad, offsets, comp_stride, d->l_size,
+                                            deterministic ? CEED_MI355_PLAN_DETERMINISTIC : 0, &d->hex);
       if (ierr == CEED_MI355_ERROR_UNSUPPORTED) ok = 0;
       else {
         CeedCallBackend(CeedMi355Check(ceed, ierr));

@@ -20,6 +20,7 @@ NOTRANSPOSE, TRANSPOSE = 0, 1
 EVAL_NONE, EVAL_INTERP, EVAL_GRAD, EVAL_WEIGHT = 0, 1, 2, 16
 QF_MASS, QF_DIFFUSION, QF_DIFFUSION_KAPPA = 0, 1, 2
 FP64, FP32 = 0, 1
+PLAN_DETERMINISTIC = 1  # colour-exclusive E^T, bitwise reproducible (default: atomic E^T)
 GAUSS, GAUSS_LOBATTO = 0, 1
 
 
@@ -143,7 +144,7 @@ class HexOperator:
     """
 
     def __init__(self, qf: int, offsets: np.ndarray, num_elem: int, P: int, Q: int, l_size: int, num_comp: int = 1,
-                 comp_stride: int | None = None, dtype=torch.float64, quad_mode: int = GAUSS):
+                 comp_stride: int | None = None, dtype=torch.float64, quad_mode: int = GAUSS, deterministic: bool = False):
         self.qf, self.P, self.Q, self.num_elem, self.num_comp = qf, P, Q, num_elem, num_comp
         self.dtype = dtype
         self.quad_mode = quad_mode
@@ -154,16 +155,18 @@ class HexOperator:
         h = C.c_void_p()
         st = FP64 if dtype == torch.float64 else FP32
         call("CeedMi355OperatorCreateHex", qf, st, num_comp, num_elem, P, Q, _ptr(self.interp), _ptr(self.grad), _ptr(off),
-             self.comp_stride, l_size, C.byref(h))
+             self.comp_stride, l_size, PLAN_DETERMINISTIC if deterministic else 0, C.byref(h))
+        self.deterministic = deterministic
         self._h = h
         self.q_comp = {QF_MASS: 1, QF_DIFFUSION: 6, QF_DIFFUSION_KAPPA: 7}[qf]
 
     @classmethod
-    def from_mesh(cls, qf: int, mesh, Q: int | None = None, num_comp: int = 1, dtype=torch.float64, build: bool = True):
+    def from_mesh(cls, qf: int, mesh, Q: int | None = None, num_comp: int = 1, dtype=torch.float64, build: bool = True,
+                  deterministic: bool = False):
         """Operator on a HexMesh with geometric factors built on the GPU from its coordinates."""
         Q = Q if Q is not None else mesh.P + 1
         op = cls(qf, mesh.offsets, mesh.num_elem, mesh.P, Q, num_comp * mesh.num_nodes, num_comp=num_comp,
-                 comp_stride=mesh.num_nodes, dtype=dtype)
+                 comp_stride=mesh.num_nodes, dtype=dtype, deterministic=deterministic)
         if build:
             coords = torch.from_numpy(mesh.coords).to("cuda")
             op.build_qdata(mesh.mesh_offsets, coords, mesh.num_vertices, mesh_P=2)

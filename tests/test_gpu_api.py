@@ -34,7 +34,8 @@ def ceed(cuda):
 def test_ceed_resource(ceed):
     assert ceed.get_resource() == "/gpu/hip/mi355"
     assert ceed.get_preferred_memtype() == libceed.MEM_DEVICE
-    assert ceed.is_deterministic()
+    assert not ceed.is_deterministic()  # fused E^T sums by atomics
+    assert libceed.Ceed("/gpu/hip/mi355:deterministic").is_deterministic()
     x, w = ceed.gauss_quadrature(5)
     assert abs(w.sum() - 2.0) < 1e-14 and np.allclose(x, -x[::-1], atol=1e-15)
 
@@ -234,6 +235,22 @@ def test_gallery_operator_fused(ceed, kind, nc, p):
     v0 = rng.uniform(-1, 1, u.size)
     v2, _ = apply_operator(ceed, m, P, Q, kind, nc, ceed.QFunctionByName(GALLERY[kind, nc]), qdata, qd_rstr, u, v0=v0)
     assert rel_l2(v2, ref + v0) < RTOL
+
+
+def test_deterministic_resource_bitwise(cuda):
+    """On /gpu/hip/mi355:deterministic the fused operator gives identical bits on every apply."""
+    c = libceed.Ceed("/gpu/hip/mi355:deterministic")
+    m = hex_mesh(5, 3, seed=8)
+    qdata, qd_rstr = build_qdata(c, m, 5, 1)
+    u = np.random.default_rng(2).uniform(-1, 1, m.num_nodes)
+    v1, op = apply_operator(c, m, 4, 5, 1, 1, c.QFunctionByName("Poisson3DApply"), qdata, qd_rstr, u)
+    uv, vv = c.Vector(m.num_nodes), c.Vector(m.num_nodes)
+    uv.set_array(u)
+    for _ in range(20):
+        op.apply(uv, vv)
+        assert np.array_equal(vv.to_numpy(), v1)
+    ref = oracle.hex_apply(1, 1, m.num_elem, 4, 5, m.offsets, m.num_nodes, oracle.hex_build_qdata(1, m, 5), u)
+    assert rel_l2(v1, ref) < RTOL
 
 
 @pytest.mark.parametrize("kind,p", [(0, 3), (1, 4), (1, 2)])

@@ -80,9 +80,11 @@ def test_cg_mixed_precision(cuda, small):
 
 @pytest.mark.parametrize("mixed", [False, True])
 def test_cg_graph_matches_eager(cuda, mixed):
+    # deterministic (colour-ordered) plans, so that eager and replayed runs can be compared bit for bit
     m = hex_mesh(6, 3, seed=2)
-    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=5)
-    solver = ops.CG(ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=5, dtype=torch.float32), op) if mixed else ops.CG(op)
+    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=5, deterministic=True)
+    op32 = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=5, dtype=torch.float32, deterministic=True)
+    solver = ops.CG(op32, op) if mixed else ops.CG(op)
     _, b = exact_rhs(m, op)
     xs, stats = [], []
     for use_graph in (False, True, True, False, True):

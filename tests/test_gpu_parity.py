@@ -163,23 +163,24 @@ def test_operator_vs_oracle(cuda, kind, nc, n, p, Q):
     u = rng.uniform(-1, 1, nc * m.num_nodes)
     qd_ref = oracle.hex_build_qdata(kind, m, Q)
     v_ref = oracle.hex_apply(kind, nc, m.num_elem, p + 1, Q, m.offsets, m.num_nodes, qd_ref, u)
-    # geometric factors built on the GPU
-    op = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc)
-    assert rel_l2(host(op.get_qdata()), qd_ref) < RTOL64
-    v = torch.zeros_like(dev(u))
-    op.apply(dev(u), v)
-    assert rel_l2(host(v), v_ref) < RTOL64
-    # geometric factors handed over in the CEED_STRIDES_BACKEND layout
-    op2 = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, build=False)
-    op2.set_qdata(dev(qd_ref))
-    v2 = torch.zeros_like(v)
-    op2.apply(dev(u), v2)
-    assert rel_l2(host(v2), v_ref) < RTOL64
-    # ApplyAdd sums into the output
-    w0 = rng.uniform(-1, 1, u.size)
-    w = dev(w0)
-    op2.apply_add(dev(u), w)
-    assert rel_l2(host(w), w0 + v_ref) < RTOL64
+    for det in (False, True):  # atomic E^T, and the colour-ordered atomic-free E^T
+        # geometric factors built on the GPU
+        op = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, deterministic=det)
+        assert rel_l2(host(op.get_qdata()), qd_ref) < RTOL64
+        v = torch.full_like(dev(u), 7.0)  # Apply overwrites
+        op.apply(dev(u), v)
+        assert rel_l2(host(v), v_ref) < RTOL64, det
+        # geometric factors handed over in the CEED_STRIDES_BACKEND layout
+        op2 = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, build=False, deterministic=det)
+        op2.set_qdata(dev(qd_ref))
+        v2 = torch.zeros_like(v)
+        op2.apply(dev(u), v2)
+        assert rel_l2(host(v2), v_ref) < RTOL64, det
+        # ApplyAdd sums into the output
+        w0 = rng.uniform(-1, 1, u.size)
+        w = dev(w0)
+        op2.apply_add(dev(u), w)
+        assert rel_l2(host(w), w0 + v_ref) < RTOL64, det
 
 
 def test_mass_config_repeated_apply(cuda):
@@ -188,7 +189,7 @@ def test_mass_config_repeated_apply(cuda):
     u = np.random.default_rng(3).uniform(-1, 1, m.num_nodes)
     qd = oracle.hex_build_qdata(0, m, 5)
     v_ref = oracle.hex_apply(0, 1, m.num_elem, 4, 5, m.offsets, m.num_nodes, qd, u)
-    op = ops.HexOperator.from_mesh(ops.QF_MASS, m, Q=5)
+    op = ops.HexOperator.from_mesh(ops.QF_MASS, m, Q=5, deterministic=True)
     ud, v = dev(u), torch.zeros(m.num_nodes, dtype=torch.float64, device="cuda")
     op.apply(ud, v)
     first = v.clone()
@@ -196,6 +197,11 @@ def test_mass_config_repeated_apply(cuda):
         op.apply(ud, v)
     assert torch.equal(first, v)
     assert rel_l2(host(v), v_ref) < RTOL64
+    # the default (atomic E^T) plan: every one of 100 applies within the bar
+    op = ops.HexOperator.from_mesh(ops.QF_MASS, m, Q=5)
+    for _ in range(100):
+        op.apply(ud, v)
+        assert rel_l2(host(v), v_ref) < RTOL64
 
 
 @pytest.mark.parametrize("kind,p", [(1, 4), (1, 6), (0, 4)])
