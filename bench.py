@@ -10,11 +10,12 @@ inputs resident in HBM.
 Also reported: a sweep over p = 1..8 at the same size in FP64 and FP32; CG
 (200 FP64 iterations at p = 4, 8M DoFs; and time to a 1e-10 relative
 residual at 16M DoFs, p = 4 and 6, FP64 CG against FP32 CG with the residual
-replaced by the FP64 one every 25 iterations); the roofline of the fused kernel (algorithmic bytes per apply / kernel time,
+replaced by the FP64 one every 25 iterations); 256 batched uneven solves
+against one CG per mesh; the roofline of the fused kernel (algorithmic bytes per apply / kernel time,
 against 8 TB/s), and a CPU baseline: the reference's /cpu/self/avx/blocked
 (built from its sources, oracle/_ref) timed on this host, one process per core.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--p 4] [--dofs 8e6] [--no-sweep] [--no-cpu] [--no-cg]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--p 4] [--dofs 8e6] [--no-sweep] [--no-cpu] [--no-cg] [--no-batch]
 """
 
 from __future__ import annotations
@@ -163,6 +164,65 @@ def run_cg_mixed(p: int, dofs: float, k: int = 25, rtol: float = 1e-10) -> dict:
     return out
 
 
+def run_batch(count: int = 256, rtol: float = 1e-8, seed: int = 0) -> dict:
+    """Batched uneven solves (one fused plan per (operator, P, Q) group, hipGraph-captured iterations) against
+    one CG per mesh, solved one after the other."""
+    import torch
+
+    from libceed_amd import ops
+    from libceed_amd.batch import BatchCG, random_systems
+
+    systems = random_systems(count, seed=seed)
+    t0 = time.perf_counter()
+    bcg = BatchCG(systems)
+    setup = time.perf_counter() - t0
+    rng = np.random.default_rng(seed)
+    b = np.zeros(bcg.n)
+    for i, s in enumerate(systems):
+        r = bcg.range(i)
+        bi = rng.uniform(-1, 1, r.stop - r.start)
+        if s.kind == ops.QF_DIFFUSION:  # pure Neumann: project b onto the range of A
+            bi -= bi.mean()
+        b[r] = bi
+    bd = torch.from_numpy(b).cuda()
+    x = torch.zeros_like(bd)
+    bcg.solve(bd, x, max_iter=50, rtol=0.0, check_every=25)  # warm-up: graph capture
+    x.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = bcg.solve(bd, x, max_iter=20000, rtol=rtol, check_every=25)
+    sec = time.perf_counter() - t0
+    lat = sec * st["converged_at_check"] / max(1, st["num_checks"])  # every check interval costs the same work
+    dofs = int(bcg.n)
+    out = {"systems": count, "dofs_total": dofs, "groups": len(bcg.ops), "rtol": rtol, "setup_seconds": round(setup, 2),
+           "autotuned_elems_per_block": {f"{'mass' if k[0] == 0 else 'diff'}_P{k[1]}_Q{k[2]}": v[0] for k, v in bcg.tuning.items()},
+           "batched": {"seconds": round(sec, 4), "solves_per_s": round(count / sec, 2), "iterations_max": int(st["iterations"].max()),
+                       "latency_p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+                       "latency_p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+                       "max_rel_residual": float(st["rel_residual"].max())}}
+    # one CG per mesh, sequentially
+    lat1 = []
+    worst = 0.0
+    for i, s in enumerate(systems):
+        r = bcg.range(i)
+        op = ops.HexOperator.from_mesh(s.kind, s.mesh, Q=s.mesh.P + 1)
+        cg = ops.CG(op)
+        bi = bd[r].contiguous()
+        xi = torch.zeros_like(bi)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sti = cg.solve(bi, xi, max_iter=20000, rtol=rtol, check_every=25)
+        lat1.append(time.perf_counter() - t1)
+        worst = max(worst, sti.rel_residual)
+        del cg, op
+    tot = float(np.sum(lat1))
+    out["one_solve_per_mesh"] = {"seconds": round(tot, 4), "solves_per_s": round(count / tot, 2),
+                                 "latency_p50_ms": round(1e3 * float(np.percentile(lat1, 50)), 3),
+                                 "latency_p99_ms": round(1e3 * float(np.percentile(lat1, 99)), 3), "max_rel_residual": worst}
+    out["speedup_solves_per_s"] = round(out["batched"]["solves_per_s"] / out["one_solve_per_mesh"]["solves_per_s"], 3)
+    return out
+
+
 def cpu_baseline(p: int = 4, seconds: float = 10.0) -> dict | None:
     """Reference /cpu/self/avx/blocked on this host's cores (one single-threaded process per core)."""
     exe = ROOT / "oracle" / "_ref" / "ref_cpu_bench"
@@ -230,6 +290,7 @@ def main() -> None:
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 1 GiB device copy (HBM reference for bandwidth and counter scaling)")
     args = ap.parse_args()
@@ -267,6 +328,11 @@ def main() -> None:
             cg[f"to_1e-10_p{p}_16M"] = run_cg_mixed(p, 16e6)
             print(f"# cg p={p}: {cg[f'to_1e-10_p{p}_16M']}", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
+    batch = None
+    if not args.no_batch:
+        batch = run_batch()
+        print(f"# batch: {batch}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
     cpu = None if args.no_cpu else cpu_baseline(args.p)
     line = {
         "metric": "3D diffusion apply GDoF/s p=1..8 (FP64 vs FP32+refine @1e-10), % HBM peak",
@@ -292,6 +358,7 @@ def main() -> None:
         "cpu_baseline": cpu,
         "sweep": sweep,
         "cg": cg,
+        "batched_solves": batch,
     }
     print(json.dumps(line), flush=True)
 

@@ -45,8 +45,8 @@ enum { CEED_MI355_FP64 = 0, CEED_MI355_FP32 = 1 };
  * kappa-weighted Poisson3DApply for -div(kappa grad u)). */
 enum { CEED_MI355_QF_MASS = 0, CEED_MI355_QF_DIFFUSION = 1, CEED_MI355_QF_DIFFUSION_KAPPA = 2 };
 
-/* Plan flags of CeedMi355OperatorCreateHex. */
-enum { CEED_MI355_PLAN_DETERMINISTIC = 1 };
+/* E^T mode flags of CeedMi355OperatorCreateHex (default 0: transpose map). */
+enum { CEED_MI355_PLAN_COLOR = 2, CEED_MI355_PLAN_ATOMIC = 4 };
 
 /* Geometric-factor builders (gallery Mass3DBuild / Poisson3DBuild). */
 enum { CEED_MI355_BUILD_MASS = 0, CEED_MI355_BUILD_DIFFUSION = 1 };
@@ -106,15 +106,21 @@ int CeedMi355BasisDestroy(CeedMi355Basis *basis);
 /* ------------------------------------------------------------------------ */
 typedef struct CeedMi355Operator_private *CeedMi355Operator;
 
-/* Plan for a 3D tensor hex operator with a gallery pointwise D, run as one
- * fused kernel per apply.  E^T: by default hardware atomic adds (one launch
- * over all elements in element order; sums in arrival order).  With
- * CEED_MI355_PLAN_DETERMINISTIC the elements are coloured so that no two
- * elements of a colour share an L-vector entry and stored in colour order;
- * one launch per colour then adds without atomics, and results are bitwise
- * reproducible.  Supports Q_1d in {P_1d, P_1d + 1}, P_1d in 2..9.
- * scalar_type selects FP64 or FP32 storage and arithmetic for u, v and
- * q_data.  Replaces the per-operator set-up of ceed-ref-operator.c:145-210. */
+/* Plan for a 3D tensor hex operator with a gallery pointwise D, applied by
+ * one fused kernel launch.  E^T (flags):
+ *   0 (default)            the fused kernel stores each element's E-vector
+ *                          block; an ordered gather-sum over the transpose
+ *                          offset map adds each L entry's contributions in
+ *                          ascending (elem, comp, node) order: no atomics,
+ *                          bitwise reproducible, the reference's summation
+ *                          order (ceed-ref-restriction.c:220-242);
+ *   CEED_MI355_PLAN_COLOR  elements coloured so that no two of a colour share
+ *                          an L entry; one launch per colour adds in place
+ *                          (no atomics, bitwise reproducible);
+ *   CEED_MI355_PLAN_ATOMIC hardware atomic adds, sums in arrival order.
+ * Supports Q_1d in {P_1d, P_1d + 1}, P_1d in 2..9.  scalar_type selects FP64
+ * or FP32 storage and arithmetic for u, v and q_data.  Replaces the
+ * per-operator set-up of ceed-ref-operator.c:145-210. */
 int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_comp, int32_t num_elem, int32_t P_1d, int32_t Q_1d,
                                const double *interp_1d_host, const double *grad_1d_host, const int32_t *offsets_host, int32_t comp_stride,
                                int64_t l_size, int flags, CeedMi355Operator *op);
@@ -134,10 +140,31 @@ int CeedMi355OperatorSetQData(CeedMi355Operator op, const void *q_data, void *st
 int CeedMi355OperatorGetQData(CeedMi355Operator op, void *q_data, void *stream);
 /* kappa at quadrature points, [elem][qpt] FP64 (QF_DIFFUSION_KAPPA only). */
 int CeedMi355OperatorSetKappa(CeedMi355Operator op, const double *kappa, void *stream);
+/* Adjoint of a QF_DIFFUSION_KAPPA plan (FP64, scalar), one fused kernel:
+ *   v = A(kappa)^T lambda                       (overwritten; A is symmetric)
+ *   dkappa[e][q] = grad(lambda)^T S grad(u)     at every quadrature point,
+ * S = w det J J^-1 J^-T, the derivative of lambda^T A(kappa) u in kappa_q;
+ * dkappa uses the [elem][Q^3] layout of CeedMi355OperatorSetKappa.  For a
+ * loss L(A(kappa) u), lambda = dL/d(Au) gives dL/du = v and dL/dkappa = dkappa.
+ * The transpose apply is the reference's CeedOperatorApply of the same
+ * operator (A symmetric; ceed-ref-operator.c:381-470). */
+int CeedMi355OperatorApplyAdjoint(CeedMi355Operator op, const double *u, const double *lambda, double *v, double *dkappa, void *stream);
+/* Diagonal of the plan's operator into the L-vector diag (plan scalar type);
+ * add != 0 sums into diag.  Replaces CeedOperatorLinearAssembleDiagonal /
+ * AddDiagonal (interface/ceed-preconditioning.c:2395-2480) for this operator. */
+int CeedMi355OperatorAssembleDiagonal(CeedMi355Operator op, void *diag, int add, void *stream);
 /* v += A u  (CeedOperatorApplyAdd, interface/ceed-operator.c:2313-2340). */
 int CeedMi355OperatorApplyAdd(CeedMi355Operator op, const void *u, void *v, void *stream);
 /* v  = A u  (CeedOperatorApply, interface/ceed-operator.c:2271-2294). */
 int CeedMi355OperatorApply(CeedMi355Operator op, const void *u, void *v, void *stream);
+/* Elements per workgroup of the fused kernel (1 <= epb, epb * Q^2 <= 256
+ * threads, LDS <= 64 KiB). */
+int CeedMi355OperatorSetLaunch(CeedMi355Operator op, int32_t elems_per_block);
+/* Autotuner: time Apply (reps times) for every admissible elements-per-
+ * workgroup value on scratch vectors, keep the fastest in the plan and
+ * remember it for later plans of the same (P, Q, num_comp, pointwise
+ * operator, scalar type, E^T mode). */
+int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int32_t *best_epb_host, double *best_ms_host, void *stream);
 /* Number of element colours, and the element count of each (host array of at
  * least num_colors entries, may be NULL). */
 int CeedMi355OperatorGetColors(CeedMi355Operator op, int32_t *num_colors, int32_t *color_sizes_host);
@@ -215,6 +242,33 @@ int CeedMi355CGCreate(CeedMi355Operator op, CeedMi355Operator op64, CeedMi355CG 
 int CeedMi355CGSolve(CeedMi355CG cg, const double *b, double *x, int32_t max_iter, double rtol, int32_t check_every, int use_graph, void *stream,
                      CeedMi355CGStats *stats_host);
 int CeedMi355CGDestroy(CeedMi355CG *cg);
+
+/* ------------------------------------------------------------------------ */
+/* Batched CG over independent systems                                       */
+/* ------------------------------------------------------------------------ */
+typedef struct CeedMi355BatchCG_private *CeedMi355BatchCG;
+
+/* num_systems independent SPD systems on one concatenated vector: system s
+ * owns entries [sys_begin[s], sys_begin[s+1]) (sys_begin has num_systems + 1
+ * entries, sys_begin[0] = 0).  The block-diagonal operator is the list of FP64
+ * plans ops[i], plan i acting on entries [op_begin[i], op_begin[i] +
+ * l_size_i); the plan ranges tile the vector, and a plan may hold the
+ * elements of several systems (one plan per (operator, P, Q) group).  No
+ * counterpart in the reference, whose operators are applied one at a time;
+ * this batches the CG solves of the benchmark problems
+ * (examples/petsc/bps.c:210-239) over many meshes. */
+int CeedMi355BatchCGCreate(int32_t num_ops, const CeedMi355Operator *ops_host, const int64_t *op_begin_host, int32_t num_systems,
+                           const int64_t *sys_begin_host, CeedMi355BatchCG *cg);
+/* Solve every system from the given x until ||r_s|| <= rtol ||b_s|| (per
+ * system; a converged system is frozen) or max_iter iterations.  Checks every
+ * check_every iterations; use_graph captures check_every iterations in a
+ * hipGraph.  Per system: iterations run, the true relative residual, and the
+ * index of the check (0 = before the first iteration) at which it was first
+ * seen converged (-1 if never). */
+int CeedMi355BatchCGSolve(CeedMi355BatchCG cg, const double *b, double *x, int32_t max_iter, double rtol, int32_t check_every, int use_graph,
+                          void *stream, int32_t *iterations_host, double *rel_residual_host, int32_t *converged_at_check_host,
+                          int32_t *num_checks_host);
+int CeedMi355BatchCGDestroy(CeedMi355BatchCG *cg);
 
 #ifdef __cplusplus
 }

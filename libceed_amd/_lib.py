@@ -36,6 +36,8 @@ SIGNATURES = {
     "CeedMi355OperatorGetQData": (c_int, [vp, vp, vp]),
     "CeedMi355OperatorSetKappa": (c_int, [vp, vp, vp]),
     "CeedMi355OperatorApplyAdd": (c_int, [vp, vp, vp, vp]),
+    "CeedMi355OperatorApplyAdjoint": (c_int, [vp, vp, vp, vp, vp, vp]),
+    "CeedMi355OperatorAssembleDiagonal": (c_int, [vp, vp, c_int, vp]),
     "CeedMi355OperatorApply": (c_int, [vp, vp, vp, vp]),
     "CeedMi355OperatorGetColors": (c_int, [vp, C.POINTER(c_i32), vp]),
     "CeedMi355OperatorDestroy": (c_int, [C.POINTER(vp)]),
@@ -51,6 +53,11 @@ SIGNATURES = {
     "CeedMi355CGCreate": (c_int, [vp, vp, C.POINTER(vp)]),
     "CeedMi355CGSolve": (c_int, [vp, vp, vp, c_i32, C.c_double, c_i32, c_int, vp, vp]),
     "CeedMi355CGDestroy": (c_int, [C.POINTER(vp)]),
+    "CeedMi355BatchCGCreate": (c_int, [c_i32, vp, vp, c_i32, vp, C.POINTER(vp)]),
+    "CeedMi355BatchCGSolve": (c_int, [vp, vp, vp, c_i32, C.c_double, c_i32, c_int, vp, vp, vp, vp, vp]),
+    "CeedMi355BatchCGDestroy": (c_int, [C.POINTER(vp)]),
+    "CeedMi355OperatorSetLaunch": (c_int, [vp, c_i32]),
+    "CeedMi355OperatorAutotune": (c_int, [vp, c_i32, vp, vp, vp]),
     "CeedRegister_Mi355": (c_int, []),
     "CeedInit_Mi355": (c_int, [C.c_char_p, vp]),
 }

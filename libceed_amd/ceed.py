@@ -104,6 +104,10 @@ _API = {
     "CeedOperatorApply": (_p, _p, _p, _p),
     "CeedOperatorApplyAdd": (_p, _p, _p, _p),
     "CeedOperatorDestroy": (C.POINTER(_p),),
+    "CeedOperatorCreateComposite": (_p, C.POINTER(_p)),
+    "CeedOperatorCompositeAddSub": (_p, _p),
+    "CeedOperatorLinearAssembleDiagonal": (_p, _p, _p),
+    "CeedOperatorLinearAssembleAddDiagonal": (_p, _p, _p),
 }
 
 
@@ -213,6 +217,9 @@ class Ceed:
 
     def Operator(self, qf, dqf=None, dqfT=None):
         return Operator(self, qf, dqf, dqfT)
+
+    def CompositeOperator(self):
+        return CompositeOperator(self)
 
 
 def registry() -> list[str]:
@@ -573,3 +580,24 @@ class Operator(_Object):
     def apply_add(self, u, v):
         up = VECTOR_NONE._pointer if u is None else u._pointer
         self._ceed._call("CeedOperatorApplyAdd", self._pointer, up, v._pointer, None)
+
+    def linear_assemble_diagonal(self, d):
+        """python/ceed_operator.py:54"""
+        self._ceed._call("CeedOperatorLinearAssembleDiagonal", self._pointer, d._pointer, None)
+
+    def linear_assemble_add_diagonal(self, d):
+        self._ceed._call("CeedOperatorLinearAssembleAddDiagonal", self._pointer, d._pointer, None)
+
+
+class CompositeOperator(Operator):
+    """Sum of sub-operators (python/ceed_operator.py:327)."""
+
+    def __init__(self, ceed):
+        self._ceed = ceed
+        self._pointer = _p()
+        self._refs = []
+        ceed._call("CeedOperatorCreateComposite", ceed._pointer, C.byref(self._pointer))
+
+    def add_sub(self, subop: Operator):
+        self._refs.append(subop)
+        self._ceed._call("CeedOperatorCompositeAddSub", self._pointer, subop._pointer)

@@ -1,6 +1,8 @@
 // Fused per-element hex operator kernels for gfx950:
 //   gather (E) -> sum-factorised interp / collocated grad (B) -> pointwise D
-//   -> B^T -> non-atomic scatter-add (E^T, element colours never share a node).
+//   -> B^T -> E^T without atomics: either the element's E-vector block is
+//   stored and an ordered gather-sum over the transpose offset map finishes
+//   E^T, or elements run in colours that never share a node and add in place.
 //
 // Thread layout: each element owns a T1 x T1 slab of threads (T1 = Q >= P),
 // thread (tx, ty) keeps the z-column of its (x, y) point in registers.  The
@@ -21,8 +23,10 @@ namespace mi355 {
 
 enum { kQfMass = 0, kQfDiffusion = 1, kQfDiffusionKappa = 2 };
 // E^T modes: colour-exclusive read-modify-write (ApplyAdd), the same with the
-// first-touch overwrite (Apply), or hardware atomics (one launch, any order)
-enum { kScatterAdd = 0, kScatterOverwrite = 1, kScatterAtomic = 2 };
+// first-touch overwrite (Apply), hardware atomics (one launch, any order), or
+// store of the element's E-vector block (one launch; a separate ordered
+// gather-sum over the transpose offset map completes E^T)
+enum { kScatterAdd = 0, kScatterOverwrite = 1, kScatterAtomic = 2, kScatterEvec = 3 };
 
 template <int QF>
 struct QfTraits;
@@ -235,6 +239,10 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
         T s = 0;
 #pragma unroll
         for (int a = 0; a < Q; a++) s += sB[a * P + tx] * S1[sidx(z, ty, a)];
+        if (scatter == kScatterEvec) {  // E-vector layout [elem][comp][node]
+          v[(e * NCOMP + comp) * (int64_t)P3 + (z * P + ty) * P + tx] = s;
+          continue;
+        }
         T *dst = &v[(off[z] >> 1) + coff];
         if (scatter == kScatterAtomic) unsafeAtomicAdd(dst, s);                        // all elements in one launch
         else if (scatter == kScatterOverwrite && (off[z] & 1)) *dst = s;  // first element to reach this entry: CeedOperatorApply writes

@@ -14,6 +14,8 @@ struct CeedMi355Operator_private;
 namespace mi355 {
 
 using ApplyFn = int (*)(const CeedMi355Operator_private *op, const void *u, void *v, int overwrite, hipStream_t s);
+// v (=|+=) E^T evec by the plan's transpose map (operator_hex.hip)
+int transpose_gather(const CeedMi355Operator_private *op, void *v, int overwrite, hipStream_t s);
 
 // Kernel launcher for one (P, Q, num_comp, QF, scalar type); nullptr when no instantiation exists.
 ApplyFn pick_apply_f64(int P, int Q, int num_comp, int qf);
@@ -30,7 +32,11 @@ struct CeedMi355Operator_private {
   int64_t l_size;
   int     epb;
   bool    covers_all = false;  // every L entry is reached by some element: Apply needs no zero fill
-  bool    atomic     = false;  // one colour, E^T by hardware atomics (not bitwise reproducible)
+  int     scatter    = 0;      // CEED_MI355_PLAN_* E^T mode: 0 transpose map, COLOR, ATOMIC
+  // transpose offset map (scatter == 0): L entry t sums E-vector positions
+  // d_tcol[d_trow[t] .. d_trow[t+1]) in ascending (elem, comp, node) order
+  int32_t *d_trow = nullptr, *d_tcol = nullptr;
+  void    *d_evec = nullptr;  // [elem][comp][P^3] element contributions, plan scalar type
   // colour c owns packed elements [color_start[c], color_start[c+1])
   std::vector<int64_t> color_start;
   std::vector<int32_t> perm;  // packed position -> caller's element index
@@ -48,15 +54,18 @@ int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v
   const int    nt  = Q * Q;
   const int    epb = op->epb;
   const size_t lds = sizeof(T) * (size_t)hex_lds_entries<P, Q, QF>(epb);
+  const bool tmap = op->scatter == 0;
   for (size_t c = 0; c + 1 < op->color_start.size(); c++) {
     const int64_t b = op->color_start[c], e = op->color_start[c + 1];
     if (e <= b) continue;
-    const int64_t grid = (e - b + epb - 1) / epb;
-    const int scatter = op->atomic ? kScatterAtomic : (overwrite ? kScatterOverwrite : kScatterAdd);
+    const int64_t grid    = (e - b + epb - 1) / epb;
+    const int     scatter = tmap ? kScatterEvec
+                                 : (op->scatter == CEED_MI355_PLAN_ATOMIC ? kScatterAtomic : (overwrite ? kScatterOverwrite : kScatterAdd));
     hex_apply_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, scatter, op->d_offsets, op->comp_stride, (const T *)op->d_qdata,
-                                                                          (const T *)op->d_mats, (const T *)u, (T *)v);
+                                                                          (const T *)op->d_mats, (const T *)u, tmap ? (T *)op->d_evec : (T *)v);
   }
   MI355_LAUNCH_CHECK();
+  if (tmap) return transpose_gather(op, v, overwrite, s);
   return CEED_MI355_SUCCESS;
 }
 

@@ -4,6 +4,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "hex_plan.hpp"
@@ -204,11 +207,74 @@ int color_elements(const int32_t *offsets, int64_t num_elem, int es, int nc, int
   return CEED_MI355_SUCCESS;
 }
 
+// Autotuner results: elements per workgroup by (P, Q, num_comp, pointwise operator, scalar type, E^T mode).
+std::mutex                                                     g_tune_mutex;
+std::map<std::tuple<int, int, int, int, int, int>, int>        g_tuned;
+
+std::tuple<int, int, int, int, int, int> tune_key(const CeedMi355Operator op) {
+  return std::make_tuple(op->P, op->Q, op->num_comp, op->qf, op->scalar_type, op->scatter);
+}
+
+int tuned_epb(const CeedMi355Operator op) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  auto                        it = g_tuned.find(tune_key(op));
+  return it == g_tuned.end() ? 0 : it->second;
+}
+
+bool admissible_epb(const CeedMi355Operator op, int epb) {
+  return epb >= 1 && epb * op->Q * op->Q <= 256 && mi355::lds_bytes(op->P, op->Q, op->qf, epb, op->scalar_bytes) <= 64 * 1024;
+}
+
 int64_t elem_block(const CeedMi355Operator op) {
   return (int64_t)op->q_comp * op->Q * op->Q * op->Q;
 }
 
+// Transpose offset map of the E-vector [elem][comp][node] (element order =
+// caller order): for every L entry, its E positions in ascending order.
+int build_transpose_map(CeedMi355Operator op, const int32_t *offsets) {
+  const int64_t es = (int64_t)op->P * op->P * op->P, nnz = (int64_t)op->num_elem * op->num_comp * es;
+  MI355_CHECK(nnz < INT32_MAX && op->l_size < INT32_MAX, CEED_MI355_ERROR_UNSUPPORTED,
+              "transpose map needs fewer than 2^31 E-vector entries (have %lld); use CEED_MI355_PLAN_COLOR", (long long)nnz);
+  std::vector<int32_t> row((size_t)op->l_size + 1, 0), col((size_t)nnz + 1);
+  for (int64_t e = 0; e < op->num_elem; e++)
+    for (int64_t k = 0; k < op->num_comp; k++)
+      for (int64_t i = 0; i < es; i++) row[(size_t)(offsets[e * es + i] + k * op->comp_stride) + 1]++;
+  for (int64_t t = 0; t < op->l_size; t++) row[t + 1] += row[t];
+  std::vector<int32_t> cursor(row.begin(), row.end() - 1);
+  for (int64_t e = 0; e < op->num_elem; e++)
+    for (int64_t k = 0; k < op->num_comp; k++)
+      for (int64_t i = 0; i < es; i++) col[cursor[offsets[e * es + i] + k * op->comp_stride]++] = (int32_t)((e * op->num_comp + k) * es + i);
+  MI355_HIP(hipMalloc(&op->d_trow, sizeof(int32_t) * row.size()));
+  MI355_HIP(hipMalloc(&op->d_tcol, sizeof(int32_t) * col.size()));
+  MI355_HIP(hipMalloc(&op->d_evec, (size_t)op->scalar_bytes * (nnz + 1)));
+  MI355_HIP(hipMemcpy(op->d_trow, row.data(), sizeof(int32_t) * row.size(), hipMemcpyHostToDevice));
+  MI355_HIP(hipMemcpy(op->d_tcol, col.data(), sizeof(int32_t) * col.size(), hipMemcpyHostToDevice));
+  return CEED_MI355_SUCCESS;
+}
+
+// v[t] (= or +=) sum of evec over row t of the transpose map, in map order
+template <typename T>
+__global__ void __launch_bounds__(256) transpose_gather_kernel(int64_t l_size, const int32_t *__restrict__ row, const int32_t *__restrict__ col,
+                                                               const T *__restrict__ evec, T *__restrict__ v, int overwrite) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < l_size; t += (int64_t)gridDim.x * blockDim.x) {
+    T acc = overwrite ? T(0) : v[t];
+    for (int32_t m = row[t]; m < row[t + 1]; m++) acc += evec[col[m]];
+    v[t] = acc;
+  }
+}
+
 }  // namespace
+
+namespace mi355 {
+int transpose_gather(const CeedMi355Operator_private *op, void *v, int overwrite, hipStream_t s) {
+  const int nb = mi355_blocks(op->l_size, 256, 1 << 20);
+  if (op->scalar_type == CEED_MI355_FP64)
+    transpose_gather_kernel<double><<<nb, 256, 0, s>>>(op->l_size, op->d_trow, op->d_tcol, (const double *)op->d_evec, (double *)v, overwrite);
+  else transpose_gather_kernel<float><<<nb, 256, 0, s>>>(op->l_size, op->d_trow, op->d_tcol, (const float *)op->d_evec, (float *)v, overwrite);
+  MI355_LAUNCH_CHECK();
+  return CEED_MI355_SUCCESS;
+}
+}  // namespace mi355
 
 extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_comp, int32_t num_elem, int32_t P_1d, int32_t Q_1d,
                                           const double *interp_1d_host, const double *grad_1d_host, const int32_t *offsets_host,
@@ -233,7 +299,12 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
   op->comp_stride = comp_stride;
   op->l_size = l_size;
   op->apply = fn;
-  op->epb = mi355::default_epb(P_1d, Q_1d, qf, op->scalar_bytes);
+  op->scatter = (flags & CEED_MI355_PLAN_ATOMIC) ? CEED_MI355_PLAN_ATOMIC : ((flags & CEED_MI355_PLAN_COLOR) ? CEED_MI355_PLAN_COLOR : 0);
+  op->epb     = mi355::default_epb(P_1d, Q_1d, qf, op->scalar_bytes);
+  {
+    const int tuned = tuned_epb(op);
+    if (tuned > 0) op->epb = tuned;
+  }
   if (const char *env = getenv("CEED_MI355_EPB")) {
     int v = atoi(env);
     if (v >= 1 && mi355::lds_bytes(P_1d, Q_1d, qf, v, op->scalar_bytes) <= 64 * 1024 && v * Q_1d * Q_1d <= 256) op->epb = v;
@@ -242,9 +313,9 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
   const int            es = P_1d * P_1d * P_1d;
   std::vector<int32_t> color;
   int                  num_colors = 0;
-  op->atomic = !(flags & CEED_MI355_PLAN_DETERMINISTIC);
-  int ierr = op->atomic ? CEED_MI355_SUCCESS : color_elements(offsets_host, num_elem, es, num_comp, comp_stride, l_size, color, num_colors);
-  if (op->atomic) {  // one colour in element order
+  const bool colored = op->scatter == CEED_MI355_PLAN_COLOR;
+  int ierr = colored ? color_elements(offsets_host, num_elem, es, num_comp, comp_stride, l_size, color, num_colors) : CEED_MI355_SUCCESS;
+  if (!colored) {  // one launch over all elements in element order
     color.assign((size_t)num_elem, 0);
     num_colors = num_elem > 0 ? 1 : 0;
   }
@@ -275,7 +346,7 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
     for (int64_t p = 0; p < num_elem; p++)
       for (int i = 0; i < es; i++) {
         const int32_t node = offsets_host[(int64_t)op->perm[p] * es + i];
-        packed[p * es + i] = 2 * node + (touched[node] || op->atomic ? 0 : 1);
+        packed[p * es + i] = 2 * node + (touched[node] || !colored ? 0 : 1);
         touched[node]      = 1;
       }
     std::vector<uint8_t> hit(l_size, 0);
@@ -289,7 +360,15 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
           }
           h = 1;
         }
-    op->covers_all = !op->atomic && std::all_of(hit.begin(), hit.end(), [](uint8_t h) { return h != 0; });
+    // the transpose map writes every L entry; atomics need a zeroed v; colours overwrite what they reach
+    op->covers_all = op->scatter == 0 || (colored && std::all_of(hit.begin(), hit.end(), [](uint8_t h) { return h != 0; }));
+  }
+  if (op->scatter == 0) {
+    ierr = build_transpose_map(op, offsets_host);
+    if (ierr) {
+      CeedMi355OperatorDestroy(&op);
+      return ierr;
+    }
   }
 
   // B and the collocated derivative D = G pinv(B), in the plan's scalar type
@@ -433,6 +512,65 @@ extern "C" int CeedMi355OperatorApply(CeedMi355Operator op, const void *u, void 
   return op->apply(op, u, v, 1, s);
 }
 
+extern "C" int CeedMi355OperatorSetLaunch(CeedMi355Operator op, int32_t elems_per_block) {
+  MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
+  MI355_CHECK(admissible_epb(op, elems_per_block), CEED_MI355_ERROR_BACKEND,
+              "%d elements per workgroup do not fit (at most 256 threads and 64 KiB of LDS for P=%d Q=%d)", elems_per_block, op->P, op->Q);
+  op->epb = elems_per_block;
+  return CEED_MI355_SUCCESS;
+}
+
+namespace {
+template <typename T>
+__global__ void __launch_bounds__(256) tune_fill(int64_t n, T *__restrict__ x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = T(1) + T(i % 7) * T(0.125);
+}
+}  // namespace
+
+extern "C" int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int32_t *best_epb, double *best_ms, void *stream) {
+  MI355_CHECK(op && reps >= 1, CEED_MI355_ERROR_MAJOR, "bad autotune arguments");
+  hipStream_t s = (hipStream_t)stream;
+  void       *u = nullptr, *v = nullptr;
+  hipEvent_t  t0, t1;
+  MI355_HIP(hipMalloc(&u, (size_t)op->scalar_bytes * (op->l_size + 1)));
+  MI355_HIP(hipMalloc(&v, (size_t)op->scalar_bytes * (op->l_size + 1)));
+  MI355_HIP(hipEventCreate(&t0));
+  MI355_HIP(hipEventCreate(&t1));
+  if (op->scalar_type == CEED_MI355_FP64) tune_fill<double><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, (double *)u);
+  else tune_fill<float><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, (float *)u);
+  const int start = op->epb;
+  int       best  = start;
+  float     bms   = 1e30f;
+  int       err   = CEED_MI355_SUCCESS;
+  for (int epb = 1; epb * op->Q * op->Q <= 256 && !err; epb++) {
+    if (!admissible_epb(op, epb)) continue;
+    op->epb = epb;
+    err     = CeedMi355OperatorApply(op, u, v, s);  // warm-up
+    if (err) break;
+    hipEventRecord(t0, s);
+    for (int r = 0; r < reps && !err; r++) err = CeedMi355OperatorApply(op, u, v, s);
+    hipEventRecord(t1, s);
+    hipEventSynchronize(t1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, t0, t1);
+    ms /= reps;
+    if (ms < bms) bms = ms, best = epb;
+  }
+  op->epb = err ? start : best;
+  hipEventDestroy(t0);
+  hipEventDestroy(t1);
+  hipFree(u);
+  hipFree(v);
+  if (err) return err;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tuned[tune_key(op)] = best;
+  }
+  if (best_epb) *best_epb = best;
+  if (best_ms) *best_ms = bms;
+  return CEED_MI355_SUCCESS;
+}
+
 extern "C" int CeedMi355OperatorGetColors(CeedMi355Operator op, int32_t *num_colors, int32_t *color_sizes_host) {
   MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
   *num_colors = (int32_t)op->color_start.size() - 1;
@@ -447,6 +585,9 @@ extern "C" int CeedMi355OperatorDestroy(CeedMi355Operator *op) {
   hipFree((*op)->d_perm);
   hipFree((*op)->d_mats);
   hipFree((*op)->d_qdata);
+  hipFree((*op)->d_trow);
+  hipFree((*op)->d_tcol);
+  hipFree((*op)->d_evec);
   delete *op;
   *op = nullptr;
   return CEED_MI355_SUCCESS;

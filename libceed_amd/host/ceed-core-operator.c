@@ -38,6 +38,7 @@ int CeedOperatorSetField(CeedOperator op, const char *field_name, CeedElemRestri
   CeedQFunction     qf = op->qf;
   CeedQFunctionField qf_field = NULL;
   struct CeedOperatorField_private *slot = NULL;
+  CeedCheck(!op->is_composite, op->ceed, CEED_ERROR_INCOMPATIBLE, "cannot add a field to a composite operator");
   CeedCheck(!op->ready, op->ceed, CEED_ERROR_MAJOR, "operator cannot be changed after it is applied");
   for (CeedInt i = 0; i < qf->num_in; i++)
     if (!strcmp(qf->in[i].name, field_name)) qf_field = &qf->in[i], slot = &op->in[i];
@@ -89,8 +90,72 @@ int CeedOperatorSetField(CeedOperator op, const char *field_name, CeedElemRestri
   return CEED_ERROR_SUCCESS;
 }
 
+/* Composite operators: the sum of their sub-operators (interface/ceed-operator.c:863-1440). */
+int CeedOperatorCreateComposite(Ceed ceed, CeedOperator *op) {
+  CeedCall(CeedCalloc(1, op));
+  CeedOperator o = *op;
+  CeedCall(CeedReferenceCopy(ceed, &o->ceed));
+  o->ref_count    = 1;
+  o->is_composite = true;
+  return CEED_ERROR_SUCCESS;
+}
+
+int CeedOperatorCompositeAddSub(CeedOperator composite_op, CeedOperator sub_op) {
+  CeedCheck(composite_op->is_composite, composite_op->ceed, CEED_ERROR_MINOR, "CeedOperator is not a composite operator");
+  CeedCheck(!sub_op->is_composite, composite_op->ceed, CEED_ERROR_UNSUPPORTED, "nested composite operators are not supported");
+  CeedCheck(!composite_op->ready, composite_op->ceed, CEED_ERROR_MAJOR, "composite operator cannot be changed after it is applied");
+  if (composite_op->num_sub > 0 && sub_op->has_num_elem) {
+    CeedSize in0, out0, in1, out1;
+    CeedCall(CeedOperatorGetActiveVectorLengths(composite_op->sub[0], &in0, &out0));
+    CeedCall(CeedOperatorGetActiveVectorLengths(sub_op, &in1, &out1));
+    CeedCheck((in0 < 0 || in1 < 0 || in0 == in1) && (out0 < 0 || out1 < 0 || out0 == out1), composite_op->ceed, CEED_ERROR_MAJOR,
+              "sub-operators must have compatible dimensions");
+  }
+  CeedCall(CeedRealloc(composite_op->num_sub + 1, &composite_op->sub));
+  composite_op->sub[composite_op->num_sub] = NULL;
+  CeedCall(CeedOperatorReferenceCopy(sub_op, &composite_op->sub[composite_op->num_sub]));
+  composite_op->num_sub++;
+  return CEED_ERROR_SUCCESS;
+}
+
+int CeedOperatorCompositeGetNumSub(CeedOperator op, CeedInt *num_suboperators) {
+  CeedCheck(op->is_composite, op->ceed, CEED_ERROR_MINOR, "CeedOperator is not a composite operator");
+  *num_suboperators = op->num_sub;
+  return CEED_ERROR_SUCCESS;
+}
+
+int CeedOperatorCompositeGetSubList(CeedOperator op, CeedOperator **sub_operators) {
+  CeedCheck(op->is_composite, op->ceed, CEED_ERROR_MINOR, "CeedOperator is not a composite operator");
+  *sub_operators = op->sub;
+  return CEED_ERROR_SUCCESS;
+}
+
+/* L-vector lengths of the active input and output (-1 if there is none) (interface/ceed-operator.c:1513). */
+int CeedOperatorGetActiveVectorLengths(CeedOperator op, CeedSize *input_size, CeedSize *output_size) {
+  *input_size = *output_size = -1;
+  if (op->is_composite) {
+    for (CeedInt i = 0; i < op->num_sub; i++) {
+      CeedSize a, b;
+      CeedCall(CeedOperatorGetActiveVectorLengths(op->sub[i], &a, &b));
+      if (a >= 0) *input_size = a;
+      if (b >= 0) *output_size = b;
+    }
+    return CEED_ERROR_SUCCESS;
+  }
+  for (CeedInt i = 0; op->qf && i < op->qf->num_in; i++)
+    if (op->in[i].vec == CEED_VECTOR_ACTIVE) *input_size = op->in[i].rstr->l_size;
+  for (CeedInt i = 0; op->qf && i < op->qf->num_out; i++)
+    if (op->out[i].vec == CEED_VECTOR_ACTIVE) *output_size = op->out[i].rstr->l_size;
+  return CEED_ERROR_SUCCESS;
+}
+
 int CeedOperatorCheckReady(CeedOperator op) {
   if (op->ready) return CEED_ERROR_SUCCESS;
+  if (op->is_composite) {
+    for (CeedInt i = 0; i < op->num_sub; i++) CeedCall(CeedOperatorCheckReady(op->sub[i]));
+    op->ready = true;
+    return CEED_ERROR_SUCCESS;
+  }
   CeedCheck(op->num_fields_set == op->qf->num_in + op->qf->num_out, op->ceed, CEED_ERROR_INCOMPLETE, "not all operator fields set");
   CeedCheck(op->has_num_elem, op->ceed, CEED_ERROR_INCOMPLETE, "at least one restriction required");
   op->ready            = true;
@@ -110,6 +175,7 @@ int CeedOperatorGetQFunction(CeedOperator op, CeedQFunction *qf) {
 
 int CeedOperatorGetFields(CeedOperator op, CeedInt *num_input_fields, CeedOperatorField **input_fields, CeedInt *num_output_fields,
                           CeedOperatorField **output_fields) {
+  CeedCheck(!op->is_composite, op->ceed, CEED_ERROR_MINOR, "a composite operator has no fields");
   if (num_input_fields) *num_input_fields = op->qf->num_in;
   if (input_fields) *input_fields = op->in_ptr;
   if (num_output_fields) *num_output_fields = op->qf->num_out;
@@ -133,8 +199,7 @@ int CeedOperatorFieldGetVector(CeedOperatorField f, CeedVector *vec) {
 }
 
 int CeedOperatorIsComposite(CeedOperator op, bool *is_composite) {
-  (void)op;
-  *is_composite = false;
+  *is_composite = op->is_composite;
   return CEED_ERROR_SUCCESS;
 }
 
@@ -160,16 +225,47 @@ int CeedOperatorGetNumQuadraturePoints(CeedOperator op, CeedInt *num_qpts) {
 
 int CeedOperatorApply(CeedOperator op, CeedVector in, CeedVector out, CeedRequest *request) {
   CeedCall(CeedOperatorCheckReady(op));
-  if (op->Apply) return op->Apply(op, in, out, request);
+  if (!op->is_composite && op->Apply) return op->Apply(op, in, out, request);
   if (out != CEED_VECTOR_NONE) CeedCall(CeedVectorSetValue(out, 0.0));
   return CeedOperatorApplyAdd(op, in, out, request);
 }
 
 int CeedOperatorApplyAdd(CeedOperator op, CeedVector in, CeedVector out, CeedRequest *request) {
   CeedCall(CeedOperatorCheckReady(op));
+  if (op->is_composite) {  /* sum of the sub-operators, in order (interface/ceed-operator.c:2327-2333) */
+    for (CeedInt i = 0; i < op->num_sub; i++) CeedCall(CeedOperatorApplyAdd(op->sub[i], in, out, request));
+    return CEED_ERROR_SUCCESS;
+  }
   CeedCheck(op->ApplyAdd, op->ceed, CEED_ERROR_UNSUPPORTED, "backend does not support OperatorApplyAdd");
   if (op->num_elem == 0) return CEED_ERROR_SUCCESS;
   return op->ApplyAdd(op, in, out, request);
+}
+
+/* Diagonal assembly (interface/ceed-preconditioning.c:2395-2480): the backend's
+ * Diagonal slot, else zero then AddDiagonal; a composite sums its sub-operators. */
+int CeedOperatorLinearAssembleAddDiagonal(CeedOperator op, CeedVector assembled, CeedRequest *request) {
+  CeedSize in, out;
+  CeedCall(CeedOperatorCheckReady(op));
+  CeedCall(CeedOperatorGetActiveVectorLengths(op, &in, &out));
+  CeedCheck(in == out, op->ceed, CEED_ERROR_DIMENSION, "Operator must be square");
+  if (op->is_composite) {
+    for (CeedInt i = 0; i < op->num_sub; i++) CeedCall(CeedOperatorLinearAssembleAddDiagonal(op->sub[i], assembled, request));
+    return CEED_ERROR_SUCCESS;
+  }
+  if (op->num_elem == 0) return CEED_ERROR_SUCCESS;
+  CeedCheck(op->LinearAssembleAddDiagonal, op->ceed, CEED_ERROR_UNSUPPORTED, "backend does not support LinearAssembleAddDiagonal for this operator");
+  return op->LinearAssembleAddDiagonal(op, assembled, request);
+}
+
+int CeedOperatorLinearAssembleDiagonal(CeedOperator op, CeedVector assembled, CeedRequest *request) {
+  CeedSize in, out;
+  CeedCall(CeedOperatorCheckReady(op));
+  CeedCall(CeedOperatorGetActiveVectorLengths(op, &in, &out));
+  CeedCheck(in == out, op->ceed, CEED_ERROR_DIMENSION, "Operator must be square");
+  if (!op->is_composite && op->num_elem == 0) return CEED_ERROR_SUCCESS;
+  if (!op->is_composite && op->LinearAssembleDiagonal) return op->LinearAssembleDiagonal(op, assembled, request);
+  CeedCall(CeedVectorSetValue(assembled, 0.0));
+  return CeedOperatorLinearAssembleAddDiagonal(op, assembled, request);
 }
 
 int CeedOperatorDestroy(CeedOperator *op) {
@@ -178,6 +274,8 @@ int CeedOperatorDestroy(CeedOperator *op) {
     return CEED_ERROR_SUCCESS;
   }
   if ((*op)->Destroy) CeedCall((*op)->Destroy(*op));
+  for (CeedInt i = 0; i < (*op)->num_sub; i++) CeedCall(CeedOperatorDestroy(&(*op)->sub[i]));
+  free((*op)->sub);
   for (int i = 0; i < CEED_CORE_MAX_FIELDS; i++) {
     struct CeedOperatorField_private *f[2] = {&(*op)->in[i], &(*op)->out[i]};
     for (int k = 0; k < 2; k++) {
@@ -188,7 +286,7 @@ int CeedOperatorDestroy(CeedOperator *op) {
       CeedCall(CeedVectorDestroy(&f[k]->vec));
     }
   }
-  CeedCall(CeedQFunctionDestroy(&(*op)->qf));
+  if ((*op)->qf) CeedCall(CeedQFunctionDestroy(&(*op)->qf));
   CeedCall(CeedDestroy(&(*op)->ceed));
   free(*op);
   *op = NULL;

@@ -357,6 +357,8 @@ static const SlotEntry slots[] = {
     SLOT(QFunctionContext, CeedQFunctionContext_private, Destroy),
     SLOT(Operator, CeedOperator_private, Apply),
     SLOT(Operator, CeedOperator_private, ApplyAdd),
+    SLOT(Operator, CeedOperator_private, LinearAssembleDiagonal),
+    SLOT(Operator, CeedOperator_private, LinearAssembleAddDiagonal),
     SLOT(Operator, CeedOperator_private, Destroy),
 };
 #undef SLOT

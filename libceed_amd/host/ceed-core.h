@@ -131,9 +131,14 @@ struct CeedOperator_private {
   CeedInt                          num_fields_set;
   CeedInt                          num_elem, num_qpts;
   bool                             has_num_elem, has_num_qpts, ready;
+  bool                             is_composite;
+  CeedOperator                    *sub;  /* composite: sub-operators */
+  CeedInt                          num_sub;
   void                            *data;
   int (*Apply)(CeedOperator, CeedVector, CeedVector, CeedRequest *);
   int (*ApplyAdd)(CeedOperator, CeedVector, CeedVector, CeedRequest *);
+  int (*LinearAssembleDiagonal)(CeedOperator, CeedVector, CeedRequest *);
+  int (*LinearAssembleAddDiagonal)(CeedOperator, CeedVector, CeedRequest *);
   int (*Destroy)(CeedOperator);
 };
 

@@ -547,9 +547,7 @@ int CeedInit_Mi355(const char *resource, Ceed ceed) {
   int *data = calloc(1, sizeof(int));
   *data     = device_id;
   CeedCallBackend(CeedSetData(ceed, data));
-  /* ":deterministic" selects the colour-ordered, atomic-free E^T of the fused
-   * operator (bitwise reproducible); the default sums by hardware atomics */
-  CeedCallBackend(CeedSetDeterministic(ceed, strstr(resource, ":deterministic") != NULL));
+  CeedCallBackend(CeedSetDeterministic(ceed, true)); /* every E^T sums in a fixed order, without atomics */
   CeedCallBackend(CeedSetBackendFunction(ceed, "Ceed", ceed, "GetPreferredMemType", GetPreferredMemType));
   CeedCallBackend(CeedSetBackendFunction(ceed, "Ceed", ceed, "Destroy", CeedDestroy_Mi355));
   CeedCallBackend(CeedSetBackendFunction(ceed, "Ceed", ceed, "VectorCreate", VectorCreate));

@@ -5,9 +5,9 @@
  * (MassApply, Vector3MassApply, Poisson3DApply, Vector3Poisson3DApply), an
  * active field with a standard restriction and a 3D tensor basis used for both
  * input and output, and passive geometric factors (EVAL_NONE), runs as one
- * fused kernel: v (+)= E^T B^T D B E u, with E^T by atomic adds, or, on a
- * deterministic resource, by one atomic-free launch per element colour (see
- * the CeedMi355OperatorCreateHex plan).  This is the path of the reference's
+ * fused kernel: v (+)= E^T B^T D B E u, whose E^T is completed without
+ * atomics by an ordered gather over the transpose offset map (see the
+ * CeedMi355OperatorCreateHex plan).  This is the path of the reference's
  * fused GPU backends (backends/hip-gen/ceed-hip-gen-operator.c:52-258),
  * rebuilt for gfx950.
  *
@@ -136,10 +136,8 @@ static int try_fused(CeedOperator op, OperatorData *d) {
     CeedCallBackend(CeedElemRestrictionGetOffsets(ru, CEED_MEM_HOST, &offsets));
     /* shapes the fused kernels are instantiated for; anything else takes the generic path */
     if (elem_size == P * P * P && P >= 2 && P <= 9 && (Q == P || Q == P + 1)) {
-      bool deterministic;
-      CeedCallBackend(CeedIsDeterministic(ceed, &deterministic));
-      int ierr = CeedMi355OperatorCreateHex(kind, CEED_MI355_FP64, num_comp, num_elem, P, Q, interp, grad, offsets, comp_stride, d->l_size,
-                                            deterministic ? CEED_MI355_PLAN_DETERMINISTIC : 0, &d->hex);
+      int ierr = CeedMi355OperatorCreateHex(kind, CEED_MI355_FP64, num_comp, num_elem, P, Q, interp, grad, offsets, comp_stride, d->l_size, 0,
+                                            &d->hex);
       if (ierr == CEED_MI355_ERROR_UNSUPPORTED) ok = 0;
       else {
         CeedCallBackend(CeedMi355Check(ceed, ierr));
@@ -400,6 +398,33 @@ static int OperatorApply(CeedOperator op, CeedVector in, CeedVector out, CeedReq
   return CEED_ERROR_SUCCESS;
 }
 
+/* Diagonal of the operator (Jacobi preconditioning): the fused plan's
+ * diagonal kernel; the staged path has no diagonal assembly. */
+static int assemble_diagonal(CeedOperator op, CeedVector assembled, bool add) {
+  OperatorData *d;
+  Ceed          ceed;
+  CeedScalar   *da;
+  CeedCallBackend(CeedOperatorGetData(op, &d));
+  CeedCallBackend(setup(op, d));
+  CeedCallBackend(CeedOperatorGetCeed(op, &ceed));
+  CeedCheck(d->fused, ceed, CEED_ERROR_UNSUPPORTED, "diagonal assembly needs a gallery mass or diffusion operator on 3D tensor hexes");
+  CeedCallBackend(CeedVectorGetArray(assembled, CEED_MEM_DEVICE, &da));
+  CeedCallBackend(CeedMi355Check(ceed, CeedMi355OperatorAssembleDiagonal(d->hex, da, add, NULL)));
+  CeedCallBackend(CeedVectorRestoreArray(assembled, &da));
+  CeedCallBackend(CeedDestroy(&ceed));
+  return CEED_ERROR_SUCCESS;
+}
+
+static int OperatorLinearAssembleDiagonal(CeedOperator op, CeedVector assembled, CeedRequest *request) {
+  (void)request;
+  return assemble_diagonal(op, assembled, false);
+}
+
+static int OperatorLinearAssembleAddDiagonal(CeedOperator op, CeedVector assembled, CeedRequest *request) {
+  (void)request;
+  return assemble_diagonal(op, assembled, true);
+}
+
 static int OperatorDestroy(CeedOperator op) {
   OperatorData *d;
   CeedCallBackend(CeedOperatorGetData(op, &d));
@@ -422,6 +447,8 @@ int CeedOperatorCreate_Mi355(CeedOperator op) {
   CeedCallBackend(CeedOperatorSetData(op, d));
   CeedCallBackend(CeedSetBackendFunction(ceed, "Operator", op, "Apply", OperatorApply));
   CeedCallBackend(CeedSetBackendFunction(ceed, "Operator", op, "ApplyAdd", OperatorApplyAdd));
+  CeedCallBackend(CeedSetBackendFunction(ceed, "Operator", op, "LinearAssembleDiagonal", OperatorLinearAssembleDiagonal));
+  CeedCallBackend(CeedSetBackendFunction(ceed, "Operator", op, "LinearAssembleAddDiagonal", OperatorLinearAssembleAddDiagonal));
   CeedCallBackend(CeedSetBackendFunction(ceed, "Operator", op, "Destroy", OperatorDestroy));
   CeedCallBackend(CeedDestroy(&ceed));
   return CEED_ERROR_SUCCESS;

@@ -90,3 +90,31 @@ def node_coordinates(mesh: HexMesh, gll_nodes: np.ndarray) -> np.ndarray:
 def mesh_size_for_dofs(p: int, target_dofs: float) -> int:
     """Elements per direction giving about `target_dofs` scalar nodes at degree p."""
     return max(1, int(round((target_dofs ** (1.0 / 3.0) - 1.0) / p)))
+
+
+def quadrature_points(mesh: HexMesh, q_ref: np.ndarray) -> np.ndarray:
+    """Physical coordinates [3, num_elem * Q^3] of the tensor quadrature points (x fastest within an element)."""
+    r = (np.asarray(q_ref) + 1.0) / 2.0
+    rx, ry, rz = np.meshgrid(r, r, r, indexing="ij")
+    rx, ry, rz = (a.transpose(2, 1, 0).reshape(-1) for a in (rx, ry, rz))
+    shp = np.empty((rx.size, 8))
+    for c in range(2):
+        for b in range(2):
+            for a in range(2):
+                shp[:, a + 2 * (b + 2 * c)] = (rx if a else 1 - rx) * (ry if b else 1 - ry) * (rz if c else 1 - rz)
+    nv = mesh.num_vertices
+    return np.stack([(mesh.coords[d * nv:(d + 1) * nv][mesh.mesh_offsets] @ shp.T).reshape(-1) for d in range(3)])
+
+
+def smooth_random_field(points: np.ndarray, seed: int = 0, modes: int = 16, amplitude: float = 0.5) -> np.ndarray:
+    """g(x) = sum_k c_k cos(2 pi m_k . x + phi_k), m_k integer wave vectors in [-3, 3]^3, with
+    sum |c_k| = amplitude so that |g| <= amplitude; evaluated at points [3, n]."""
+    rng = np.random.default_rng(seed)
+    m = rng.integers(-3, 4, size=(modes, 3))
+    phi = rng.uniform(0.0, 2.0 * np.pi, modes)
+    c = rng.uniform(-1.0, 1.0, modes)
+    c *= amplitude / np.abs(c).sum()
+    g = np.zeros(points.shape[1])
+    for k in range(modes):
+        g += c[k] * np.cos(2.0 * np.pi * (m[k] @ points) + phi[k])
+    return g

@@ -20,7 +20,8 @@ NOTRANSPOSE, TRANSPOSE = 0, 1
 EVAL_NONE, EVAL_INTERP, EVAL_GRAD, EVAL_WEIGHT = 0, 1, 2, 16
 QF_MASS, QF_DIFFUSION, QF_DIFFUSION_KAPPA = 0, 1, 2
 FP64, FP32 = 0, 1
-PLAN_DETERMINISTIC = 1  # colour-exclusive E^T, bitwise reproducible (default: atomic E^T)
+# E^T of the fused plan: transpose offset map (default), element colours, hardware atomics
+SCATTER_FLAGS = {"tmap": 0, "color": 2, "atomic": 4}
 GAUSS, GAUSS_LOBATTO = 0, 1
 
 
@@ -144,7 +145,7 @@ class HexOperator:
     """
 
     def __init__(self, qf: int, offsets: np.ndarray, num_elem: int, P: int, Q: int, l_size: int, num_comp: int = 1,
-                 comp_stride: int | None = None, dtype=torch.float64, quad_mode: int = GAUSS, deterministic: bool = False):
+                 comp_stride: int | None = None, dtype=torch.float64, quad_mode: int = GAUSS, scatter: str = "tmap"):
         self.qf, self.P, self.Q, self.num_elem, self.num_comp = qf, P, Q, num_elem, num_comp
         self.dtype = dtype
         self.quad_mode = quad_mode
@@ -155,18 +156,18 @@ class HexOperator:
         h = C.c_void_p()
         st = FP64 if dtype == torch.float64 else FP32
         call("CeedMi355OperatorCreateHex", qf, st, num_comp, num_elem, P, Q, _ptr(self.interp), _ptr(self.grad), _ptr(off),
-             self.comp_stride, l_size, PLAN_DETERMINISTIC if deterministic else 0, C.byref(h))
-        self.deterministic = deterministic
+             self.comp_stride, l_size, SCATTER_FLAGS[scatter], C.byref(h))
+        self.scatter = scatter
         self._h = h
         self.q_comp = {QF_MASS: 1, QF_DIFFUSION: 6, QF_DIFFUSION_KAPPA: 7}[qf]
 
     @classmethod
     def from_mesh(cls, qf: int, mesh, Q: int | None = None, num_comp: int = 1, dtype=torch.float64, build: bool = True,
-                  deterministic: bool = False):
+                  scatter: str = "tmap"):
         """Operator on a HexMesh with geometric factors built on the GPU from its coordinates."""
         Q = Q if Q is not None else mesh.P + 1
         op = cls(qf, mesh.offsets, mesh.num_elem, mesh.P, Q, num_comp * mesh.num_nodes, num_comp=num_comp,
-                 comp_stride=mesh.num_nodes, dtype=dtype, deterministic=deterministic)
+                 comp_stride=mesh.num_nodes, dtype=dtype, scatter=scatter)
         if build:
             coords = torch.from_numpy(mesh.coords).to("cuda")
             op.build_qdata(mesh.mesh_offsets, coords, mesh.num_vertices, mesh_P=2)
@@ -198,6 +199,22 @@ class HexOperator:
     def apply_add(self, u: torch.Tensor, v: torch.Tensor) -> None:
         self._check(u, v)
         call("CeedMi355OperatorApplyAdd", self._h, _ptr(u), _ptr(v), _stream())
+
+    def apply_adjoint(self, u: torch.Tensor, lam: torch.Tensor, v: torch.Tensor, dkappa: torch.Tensor) -> None:
+        """v = A(kappa)^T lam and dkappa[e, q] = grad(lam) . S grad(u) (QF_DIFFUSION_KAPPA, FP64)."""
+        self._check(u, v)
+        self._check(lam, v)
+        _device_tensor(dkappa, torch.float64)
+        if dkappa.numel() < self.num_elem * self.Q ** 3:
+            raise ValueError("dkappa needs num_elem * Q^3 entries")
+        call("CeedMi355OperatorApplyAdjoint", self._h, _ptr(u), _ptr(lam), _ptr(v), _ptr(dkappa), _stream())
+
+    def assemble_diagonal(self, diag: torch.Tensor, add: bool = False) -> None:
+        """diag (=|+=) the diagonal of the operator."""
+        _device_tensor(diag, self.dtype)
+        if diag.numel() < self.l_size:
+            raise ValueError("diagonal vector too short")
+        call("CeedMi355OperatorAssembleDiagonal", self._h, _ptr(diag), int(add), _stream())
 
     def colors(self) -> list[int]:
         n = C.c_int32()

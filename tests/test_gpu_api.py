@@ -34,8 +34,7 @@ def ceed(cuda):
 def test_ceed_resource(ceed):
     assert ceed.get_resource() == "/gpu/hip/mi355"
     assert ceed.get_preferred_memtype() == libceed.MEM_DEVICE
-    assert not ceed.is_deterministic()  # fused E^T sums by atomics
-    assert libceed.Ceed("/gpu/hip/mi355:deterministic").is_deterministic()
+    assert ceed.is_deterministic()  # every E^T sums in a fixed order, without atomics
     x, w = ceed.gauss_quadrature(5)
     assert abs(w.sum() - 2.0) < 1e-14 and np.allclose(x, -x[::-1], atol=1e-15)
 
@@ -237,9 +236,9 @@ def test_gallery_operator_fused(ceed, kind, nc, p):
     assert rel_l2(v2, ref + v0) < RTOL
 
 
-def test_deterministic_resource_bitwise(cuda):
-    """On /gpu/hip/mi355:deterministic the fused operator gives identical bits on every apply."""
-    c = libceed.Ceed("/gpu/hip/mi355:deterministic")
+def test_fused_operator_bitwise_repeatable(cuda):
+    """The fused operator gives identical bits on every apply."""
+    c = libceed.Ceed("/gpu/hip/mi355")
     m = hex_mesh(5, 3, seed=8)
     qdata, qd_rstr = build_qdata(c, m, 5, 1)
     u = np.random.default_rng(2).uniform(-1, 1, m.num_nodes)
@@ -375,3 +374,39 @@ def test_field_errors(ceed):
         op.set_field("du", rstr, vbasis, libceed.VECTOR_ACTIVE)  # 9 components vs QFunction size 3
     with pytest.raises(libceed.CeedError, match="not all operator fields"):
         op.check()
+
+
+def test_composite_operator_and_diagonal(ceed):
+    """A composite of mass and diffusion sub-operators applies their sum; its
+    assembled diagonal is the sum of the sub-operators' diagonals, each equal to
+    the diagonal of the dense operator the oracle assembles."""
+    m = hex_mesh(2, 2, seed=6)
+    P, Q = 3, 4
+    N = m.num_nodes
+    qm, qm_rstr = build_qdata(ceed, m, Q, 0)
+    qd, qd_rstr = build_qdata(ceed, m, Q, 1)
+    _, op_m = apply_operator(ceed, m, P, Q, 0, 1, ceed.QFunctionByName("MassApply"), qm, qm_rstr, np.zeros(N))
+    _, op_k = apply_operator(ceed, m, P, Q, 1, 1, ceed.QFunctionByName("Poisson3DApply"), qd, qd_rstr, np.zeros(N))
+    comp = ceed.CompositeOperator()
+    comp.add_sub(op_m)
+    comp.add_sub(op_k)
+    u = np.random.default_rng(5).uniform(-1, 1, N)
+    uv, vv = ceed.Vector(N), ceed.Vector(N)
+    uv.set_array(u)
+    comp.apply(uv, vv)
+    qm_ref, qd_ref = oracle.hex_build_qdata(0, m, Q), oracle.hex_build_qdata(1, m, Q)
+    ref = oracle.hex_apply(0, 1, m.num_elem, P, Q, m.offsets, N, qm_ref, u)
+    ref = oracle.hex_apply(1, 1, m.num_elem, P, Q, m.offsets, N, qd_ref, u, v=ref)
+    assert rel_l2(vv.to_numpy(), ref) < RTOL
+    dm, dk = np.zeros(N), np.zeros(N)
+    for j in range(N):
+        e = np.zeros(N)
+        e[j] = 1.0
+        dm[j] = oracle.hex_apply(0, 1, m.num_elem, P, Q, m.offsets, N, qm_ref, e)[j]
+        dk[j] = oracle.hex_apply(1, 1, m.num_elem, P, Q, m.offsets, N, qd_ref, e)[j]
+    dv = ceed.Vector(N)
+    dv.set_value(3.0)  # overwritten
+    comp.linear_assemble_diagonal(dv)
+    assert rel_l2(dv.to_numpy(), dm + dk) < RTOL
+    op_k.linear_assemble_add_diagonal(dv)  # adds the diffusion diagonal once more
+    assert rel_l2(dv.to_numpy(), dm + 2 * dk) < RTOL

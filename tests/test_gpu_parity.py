@@ -163,15 +163,15 @@ def test_operator_vs_oracle(cuda, kind, nc, n, p, Q):
     u = rng.uniform(-1, 1, nc * m.num_nodes)
     qd_ref = oracle.hex_build_qdata(kind, m, Q)
     v_ref = oracle.hex_apply(kind, nc, m.num_elem, p + 1, Q, m.offsets, m.num_nodes, qd_ref, u)
-    for det in (False, True):  # atomic E^T, and the colour-ordered atomic-free E^T
+    for det in ("tmap", "color", "atomic"):  # the three E^T modes of the fused plan
         # geometric factors built on the GPU
-        op = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, deterministic=det)
+        op = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, scatter=det)
         assert rel_l2(host(op.get_qdata()), qd_ref) < RTOL64
         v = torch.full_like(dev(u), 7.0)  # Apply overwrites
         op.apply(dev(u), v)
         assert rel_l2(host(v), v_ref) < RTOL64, det
         # geometric factors handed over in the CEED_STRIDES_BACKEND layout
-        op2 = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, build=False, deterministic=det)
+        op2 = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, build=False, scatter=det)
         op2.set_qdata(dev(qd_ref))
         v2 = torch.zeros_like(v)
         op2.apply(dev(u), v2)
@@ -189,16 +189,17 @@ def test_mass_config_repeated_apply(cuda):
     u = np.random.default_rng(3).uniform(-1, 1, m.num_nodes)
     qd = oracle.hex_build_qdata(0, m, 5)
     v_ref = oracle.hex_apply(0, 1, m.num_elem, 4, 5, m.offsets, m.num_nodes, qd, u)
-    op = ops.HexOperator.from_mesh(ops.QF_MASS, m, Q=5, deterministic=True)
     ud, v = dev(u), torch.zeros(m.num_nodes, dtype=torch.float64, device="cuda")
-    op.apply(ud, v)
-    first = v.clone()
-    for _ in range(99):
+    for scatter in ("tmap", "color"):  # the atomic-free E^T modes: bitwise repeatable
+        op = ops.HexOperator.from_mesh(ops.QF_MASS, m, Q=5, scatter=scatter)
         op.apply(ud, v)
-    assert torch.equal(first, v)
-    assert rel_l2(host(v), v_ref) < RTOL64
-    # the default (atomic E^T) plan: every one of 100 applies within the bar
-    op = ops.HexOperator.from_mesh(ops.QF_MASS, m, Q=5)
+        first = v.clone()
+        for _ in range(99):
+            op.apply(ud, v)
+        assert torch.equal(first, v)
+        assert rel_l2(host(v), v_ref) < RTOL64
+    # hardware atomics: every one of 100 applies within the bar
+    op = ops.HexOperator.from_mesh(ops.QF_MASS, m, Q=5, scatter="atomic")
     for _ in range(100):
         op.apply(ud, v)
         assert rel_l2(host(v), v_ref) < RTOL64
