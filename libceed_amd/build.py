@@ -91,7 +91,24 @@ def build(jobs: int | None = None, force: bool = True, verbose: bool = False) ->
     tmp = KERNELS_LIB.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, kobjs), *link])
     os.replace(tmp, KERNELS_LIB)
+    build_c_programs()
     return LIB
+
+
+def build_c_programs() -> list[Path]:
+    """C programs written against the libCEED C API (tests/c/*.c), linked to the library."""
+    src_dir = ROOT / "tests" / "c"
+    out = []
+    if not src_dir.exists():
+        return out
+    bin_dir = src_dir / "bin"
+    bin_dir.mkdir(exist_ok=True)
+    for src in sorted(src_dir.glob("*.c")):
+        exe = bin_dir / src.stem
+        _run(["gcc", "-O2", "-std=c11", "-Wall", f"-I{ROOT / 'include'}", f"-I{src_dir}", str(src.resolve()), f"-L{LIB_DIR}", "-lceed-mi355",
+              "-Wl,-rpath,$ORIGIN/../../../libceed_amd/lib", "-lm", "-o", str(exe)])
+        out.append(exe)
+    return out
 
 
 if __name__ == "__main__":

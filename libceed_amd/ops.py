@@ -216,6 +216,15 @@ class HexOperator:
             raise ValueError("diagonal vector too short")
         call("CeedMi355OperatorAssembleDiagonal", self._h, _ptr(diag), int(add), _stream())
 
+    def set_launch(self, elems_per_block: int) -> None:
+        call("CeedMi355OperatorSetLaunch", self._h, int(elems_per_block))
+
+    def autotune(self, reps: int = 5) -> tuple[int, float]:
+        """Pick the fastest elements-per-workgroup for this plan; returns (epb, ms per apply)."""
+        best, ms = C.c_int32(), C.c_double()
+        call("CeedMi355OperatorAutotune", self._h, int(reps), C.byref(best), C.byref(ms), _stream())
+        return best.value, ms.value
+
     def colors(self) -> list[int]:
         n = C.c_int32()
         call("CeedMi355OperatorGetColors", self._h, C.byref(n), C.c_void_p(0))
