@@ -45,8 +45,8 @@ enum { CEED_MI355_FP64 = 0, CEED_MI355_FP32 = 1 };
  * kappa-weighted Poisson3DApply for -div(kappa grad u)). */
 enum { CEED_MI355_QF_MASS = 0, CEED_MI355_QF_DIFFUSION = 1, CEED_MI355_QF_DIFFUSION_KAPPA = 2 };
 
-/* E^T mode flags of CeedMi355OperatorCreateHex (default 0: transpose map). */
-enum { CEED_MI355_PLAN_COLOR = 2, CEED_MI355_PLAN_ATOMIC = 4 };
+/* E^T mode flags of CeedMi355OperatorCreateHex (0: transpose map for P >= 5, colours below). */
+enum { CEED_MI355_PLAN_COLOR = 2, CEED_MI355_PLAN_ATOMIC = 4, CEED_MI355_PLAN_TMAP = 8 };
 
 /* Geometric-factor builders (gallery Mass3DBuild / Poisson3DBuild). */
 enum { CEED_MI355_BUILD_MASS = 0, CEED_MI355_BUILD_DIFFUSION = 1 };
@@ -107,13 +107,15 @@ int CeedMi355BasisDestroy(CeedMi355Basis *basis);
 typedef struct CeedMi355Operator_private *CeedMi355Operator;
 
 /* Plan for a 3D tensor hex operator with a gallery pointwise D, applied by
- * one fused kernel launch.  E^T (flags):
- *   0 (default)            the fused kernel stores each element's E-vector
- *                          block; an ordered gather-sum over the transpose
- *                          offset map adds each L entry's contributions in
- *                          ascending (elem, comp, node) order: no atomics,
- *                          bitwise reproducible, the reference's summation
- *                          order (ceed-ref-restriction.c:220-242);
+ * one fused kernel launch.  E^T (flags; 0 picks TMAP for P_1d >= 5, COLOR below):
+ *   CEED_MI355_PLAN_TMAP   the fused kernel writes entries reached by one
+ *                          element straight to v and stores the others in
+ *                          an E-vector; an ordered gather-sum over the
+ *                          transpose offset map adds each shared L entry's
+ *                          contributions in ascending (elem, comp, node)
+ *                          order: no atomics, bitwise reproducible, the
+ *                          reference's summation order
+ *                          (ceed-ref-restriction.c:220-242);
  *   CEED_MI355_PLAN_COLOR  elements coloured so that no two of a colour share
  *                          an L entry; one launch per colour adds in place
  *                          (no atomics, bitwise reproducible);

@@ -42,7 +42,7 @@ def random_systems(count: int = 256, seed: int = 0, max_n: int = 24, p_range=(2,
 class BatchCG:
     """Batched CG over a list of systems; autotune=True tunes each group's elements per workgroup."""
 
-    def __init__(self, systems: list[System], scatter: str = "tmap", autotune: bool = True):
+    def __init__(self, systems: list[System], scatter: str = "auto", autotune: bool = True):
         self.systems = systems
         groups: dict[tuple, list[int]] = {}
         for i, s in enumerate(systems):

@@ -41,10 +41,12 @@ __device__ double block_sum(double v) {
   return s[0];
 }
 
-// one block per chunk: partial[c] = sum over the chunk of a*b
-__global__ void __launch_bounds__(kThreads) k_chunk_dot(const int64_t *__restrict__ chunk_begin, const double *__restrict__ a,
+// one block per chunk: partial[c] = sum over the chunk of a*b (chunks of frozen systems: skipped)
+__global__ void __launch_bounds__(kThreads) k_chunk_dot(const int64_t *__restrict__ chunk_begin, const int32_t *__restrict__ chunk_sys,
+                                                        const int32_t *__restrict__ active, const double *__restrict__ a,
                                                         const double *__restrict__ b, double *__restrict__ partial) {
   const int64_t c = blockIdx.x, lo = chunk_begin[c], hi = chunk_begin[c + 1];
+  if (active && !active[chunk_sys[c]]) return;  // uniform per block; the finish kernel ignores frozen systems
   double        acc = 0;
   for (int64_t i = lo + threadIdx.x; i < hi; i += kThreads) acc += a[i] * b[i];
   const double s = block_sum(acc);
@@ -54,9 +56,10 @@ __global__ void __launch_bounds__(kThreads) k_chunk_dot(const int64_t *__restric
 // r -= alpha_s Ap, and the chunk partials of r.r
 __global__ void __launch_bounds__(kThreads) k_chunk_update_r(const int64_t *__restrict__ chunk_begin, const int32_t *__restrict__ chunk_sys,
                                                              const double *__restrict__ scal, double *__restrict__ r, const double *__restrict__ Ap,
-                                                             double *__restrict__ partial) {
+                                                             const int32_t *__restrict__ active, double *__restrict__ partial) {
   const int64_t c = blockIdx.x, lo = chunk_begin[c], hi = chunk_begin[c + 1];
-  const double  alpha = scal[(int64_t)chunk_sys[c] * kNumSlots + kAlpha];
+  if (!active[chunk_sys[c]]) return;
+  const double alpha = scal[(int64_t)chunk_sys[c] * kNumSlots + kAlpha];
   double        acc   = 0;
   for (int64_t i = lo + threadIdx.x; i < hi; i += kThreads) {
     const double ri = r[i] - alpha * Ap[i];
@@ -70,8 +73,9 @@ __global__ void __launch_bounds__(kThreads) k_chunk_update_r(const int64_t *__re
 // x += alpha_s p; p = r + beta_s p
 __global__ void __launch_bounds__(kThreads) k_chunk_update_px(const int64_t *__restrict__ chunk_begin, const int32_t *__restrict__ chunk_sys,
                                                               const double *__restrict__ scal, double *__restrict__ x, double *__restrict__ p,
-                                                              const double *__restrict__ r) {
+                                                              const double *__restrict__ r, const int32_t *__restrict__ was_active) {
   const int64_t c = blockIdx.x, lo = chunk_begin[c], hi = chunk_begin[c + 1];
+  if (!was_active[chunk_sys[c]]) return;
   const double *sc = scal + (int64_t)chunk_sys[c] * kNumSlots;
   const double  alpha = sc[kAlpha], beta = sc[kBeta];
   for (int64_t i = lo + threadIdx.x; i < hi; i += kThreads) {
@@ -100,17 +104,22 @@ __global__ void __launch_bounds__(kThreads) k_chunk_residual(const int64_t *__re
 // per system: the ordered sum of its chunk partials, then the scalar update
 //   mode 0: scal[slot] = sum;  mode 1: p.Ap -> alpha;  mode 2: r.r -> beta, RR, iteration count;
 //   mode 3: b.b -> BB and the stopping threshold rtol^2 b.b
+//   modes 0 and 2 also refresh active[s] = (r.r > threshold); prev[s] keeps the flag the
+//   iteration started with (the x/p update of a system that just converged still runs once)
 __global__ void __launch_bounds__(kThreads) k_sys_finish(int32_t num_sys, const int32_t *__restrict__ sys_chunk, const double *__restrict__ partial,
-                                                         double *__restrict__ scal, int32_t *__restrict__ iters, int slot, int mode, double rtol2) {
+                                                         double *__restrict__ scal, int32_t *__restrict__ iters, int32_t *__restrict__ active_flag,
+                                                         int32_t *__restrict__ prev_flag, int slot, int mode, double rtol2) {
   for (int32_t s = blockIdx.x * kThreads + threadIdx.x; s < num_sys; s += gridDim.x * kThreads) {
-    double acc = 0;
-    for (int32_t c = sys_chunk[s]; c < sys_chunk[s + 1]; c++) acc += partial[c];
     double *sc = scal + (int64_t)s * kNumSlots;
     const bool active = sc[kRR] > sc[kTol2];
+    double     acc    = 0;
+    if (mode != 1 && mode != 2 || active)
+      for (int32_t c = sys_chunk[s]; c < sys_chunk[s + 1]; c++) acc += partial[c];
     if (mode == 1) {
       sc[kPAP]   = acc;
       sc[kAlpha] = active && acc != 0.0 ? sc[kRR] / acc : 0.0;
     } else if (mode == 2) {
+      prev_flag[s] = active;
       if (active) {
         sc[kBeta] = sc[kRR] != 0.0 ? acc / sc[kRR] : 0.0;
         sc[kRR]   = acc;
@@ -118,11 +127,13 @@ __global__ void __launch_bounds__(kThreads) k_sys_finish(int32_t num_sys, const 
       } else {
         sc[kBeta] = 0.0;
       }
+      active_flag[s] = sc[kRR] > sc[kTol2];
     } else if (mode == 3) {
       sc[kBB]   = acc;
       sc[kTol2] = rtol2 * acc;
     } else {
       sc[slot] = acc;
+      if (slot == kRR) active_flag[s] = prev_flag[s] = acc > sc[kTol2];
     }
   }
 }
@@ -143,6 +154,8 @@ struct CeedMi355BatchCG_private {
   int64_t                        n       = 0;
   int64_t                       *d_chunk_begin = nullptr;
   int32_t                       *d_chunk_sys = nullptr, *d_sys_chunk = nullptr, *d_iters = nullptr, *h_iters = nullptr;
+  int32_t                       *d_active = nullptr, *d_prev = nullptr;            // per system
+  std::vector<int32_t *>         d_elem_sys;                                       // per plan: system of each packed element
   double                        *d_scal = nullptr, *h_scal = nullptr, *d_partial = nullptr;
   double                        *r = nullptr, *p = nullptr, *Ap = nullptr;
   hipStream_t                    stream = nullptr;
@@ -155,9 +168,16 @@ struct CeedMi355BatchCG_private {
 
 namespace {
 
-int apply_all(CeedMi355BatchCG cg, const double *u, double *v) {
+// A p for every plan; with skip, the elements of frozen systems are left out
+// (their stale entries of Ap are never used: a frozen system's alpha is 0)
+int apply_all(CeedMi355BatchCG cg, const double *u, double *v, bool skip = false) {
   for (size_t i = 0; i < cg->ops.size(); i++) {
-    const int err = CeedMi355OperatorApply(cg->ops[i], u + cg->op_begin[i], v + cg->op_begin[i], cg->stream);
+    CeedMi355Operator op = cg->ops[i];
+    op->d_elem_sys       = skip ? cg->d_elem_sys[i] : nullptr;
+    op->d_sys_active     = skip ? cg->d_active : nullptr;
+    const int err        = CeedMi355OperatorApply(op, u + cg->op_begin[i], v + cg->op_begin[i], cg->stream);
+    op->d_elem_sys       = nullptr;
+    op->d_sys_active     = nullptr;
     if (err) return err;
   }
   return CEED_MI355_SUCCESS;
@@ -165,7 +185,7 @@ int apply_all(CeedMi355BatchCG cg, const double *u, double *v) {
 
 int finish(CeedMi355BatchCG cg, int slot, int mode, double rtol2 = 0) {
   k_sys_finish<<<mi355_blocks(cg->num_sys, kThreads, 1024), kThreads, 0, cg->stream>>>(cg->num_sys, cg->d_sys_chunk, cg->d_partial, cg->d_scal,
-                                                                                      cg->d_iters, slot, mode, rtol2);
+                                                                                      cg->d_iters, cg->d_active, cg->d_prev, slot, mode, rtol2);
   MI355_LAUNCH_CHECK();
   return CEED_MI355_SUCCESS;
 }
@@ -174,12 +194,12 @@ int iteration(CeedMi355BatchCG cg, double *x) {
   hipStream_t s  = cg->stream;
   const int   nc = cg->num_chunks;
   int         err;
-  if ((err = apply_all(cg, cg->p, cg->Ap))) return err;
-  k_chunk_dot<<<nc, kThreads, 0, s>>>(cg->d_chunk_begin, cg->p, cg->Ap, cg->d_partial);
+  if ((err = apply_all(cg, cg->p, cg->Ap, true))) return err;
+  k_chunk_dot<<<nc, kThreads, 0, s>>>(cg->d_chunk_begin, cg->d_chunk_sys, cg->d_active, cg->p, cg->Ap, cg->d_partial);
   if ((err = finish(cg, kPAP, 1))) return err;
-  k_chunk_update_r<<<nc, kThreads, 0, s>>>(cg->d_chunk_begin, cg->d_chunk_sys, cg->d_scal, cg->r, cg->Ap, cg->d_partial);
+  k_chunk_update_r<<<nc, kThreads, 0, s>>>(cg->d_chunk_begin, cg->d_chunk_sys, cg->d_scal, cg->r, cg->Ap, cg->d_active, cg->d_partial);
   if ((err = finish(cg, kRR, 2))) return err;
-  k_chunk_update_px<<<nc, kThreads, 0, s>>>(cg->d_chunk_begin, cg->d_chunk_sys, cg->d_scal, x, cg->p, cg->r);
+  k_chunk_update_px<<<nc, kThreads, 0, s>>>(cg->d_chunk_begin, cg->d_chunk_sys, cg->d_scal, x, cg->p, cg->r, cg->d_prev);
   MI355_LAUNCH_CHECK();
   return CEED_MI355_SUCCESS;
 }
@@ -279,6 +299,23 @@ extern "C" int CeedMi355BatchCGCreate(int32_t num_ops, const CeedMi355Operator *
             hipMalloc(&cg->d_partial, sizeof(double) * chunk_sys.size()) == hipSuccess &&
             hipMalloc(&cg->r, sizeof(double) * n) == hipSuccess && hipMalloc(&cg->p, sizeof(double) * n) == hipSuccess &&
             hipMalloc(&cg->Ap, sizeof(double) * n) == hipSuccess;
+  ok = ok && hipMalloc(&cg->d_active, sizeof(int32_t) * num_systems) == hipSuccess &&
+       hipMalloc(&cg->d_prev, sizeof(int32_t) * num_systems) == hipSuccess;
+  // system of every packed element of every plan (from the element's first node)
+  for (int32_t i = 0; ok && i < num_ops; i++) {
+    CeedMi355Operator    op = ops_host[i];
+    const int64_t        es = (int64_t)op->P * op->P * op->P;
+    std::vector<int32_t> off((size_t)(op->num_elem * es)), sys((size_t)op->num_elem + 1, 0);
+    ok = op->num_elem == 0 || hipMemcpy(off.data(), op->d_offsets, sizeof(int32_t) * off.size(), hipMemcpyDeviceToHost) == hipSuccess;
+    for (int64_t e = 0; ok && e < op->num_elem; e++) {
+      const int64_t node = op_begin_host[i] + (off[(size_t)(e * es)] >> 1);
+      sys[(size_t)e]     = (int32_t)(std::upper_bound(sys_begin_host, sys_begin_host + num_systems + 1, node) - sys_begin_host - 1);
+    }
+    int32_t *d = nullptr;
+    ok = ok && hipMalloc(&d, sizeof(int32_t) * sys.size()) == hipSuccess &&
+         hipMemcpy(d, sys.data(), sizeof(int32_t) * sys.size(), hipMemcpyHostToDevice) == hipSuccess;
+    cg->d_elem_sys.push_back(d);
+  }
   ok = ok && hipMemcpy(cg->d_chunk_begin, chunk_begin.data(), sizeof(int64_t) * chunk_begin.size(), hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(cg->d_chunk_sys, chunk_sys.data(), sizeof(int32_t) * chunk_sys.size(), hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(cg->d_sys_chunk, sys_chunk.data(), sizeof(int32_t) * sys_chunk.size(), hipMemcpyHostToDevice) == hipSuccess;
@@ -301,7 +338,7 @@ extern "C" int CeedMi355BatchCGSolve(CeedMi355BatchCG cg, const double *b, doubl
   MI355_HIP(hipStreamWaitEvent(s, cg->ev, 0));
   k_zero<<<mi355_blocks((int64_t)kNumSlots * cg->num_sys, kThreads, 1024), kThreads, 0, s>>>((int64_t)kNumSlots * cg->num_sys, cg->d_scal);
   k_zero_i<<<mi355_blocks(cg->num_sys, kThreads, 1024), kThreads, 0, s>>>(cg->num_sys, cg->d_iters);
-  k_chunk_dot<<<nc, kThreads, 0, s>>>(cg->d_chunk_begin, b, b, cg->d_partial);
+  k_chunk_dot<<<nc, kThreads, 0, s>>>(cg->d_chunk_begin, cg->d_chunk_sys, nullptr, b, b, cg->d_partial);
   MI355_LAUNCH_CHECK();
   if ((err = finish(cg, kBB, 3, rtol * rtol))) return err;
   if ((err = apply_all(cg, x, cg->Ap))) return err;
@@ -358,6 +395,9 @@ extern "C" int CeedMi355BatchCGDestroy(CeedMi355BatchCG *cg) {
   hipFree(c->r);
   hipFree(c->p);
   hipFree(c->Ap);
+  hipFree(c->d_active);
+  hipFree(c->d_prev);
+  for (int32_t *d : c->d_elem_sys) hipFree(d);
   if (c->ev) hipEventDestroy(c->ev);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;

@@ -25,8 +25,10 @@ enum { kQfMass = 0, kQfDiffusion = 1, kQfDiffusionKappa = 2 };
 // E^T modes: colour-exclusive read-modify-write (ApplyAdd), the same with the
 // first-touch overwrite (Apply), hardware atomics (one launch, any order), or
 // store of the element's E-vector block (one launch; a separate ordered
-// gather-sum over the transpose offset map completes E^T)
-enum { kScatterAdd = 0, kScatterOverwrite = 1, kScatterAtomic = 2, kScatterEvec = 3 };
+// gather-sum over the transpose offset map completes E^T).  In the E-vector
+// modes, entries whose node no other element touches (offset bit 0 set) go
+// straight to v (= in Apply, += in ApplyAdd) and skip the E-vector.
+enum { kScatterAdd = 0, kScatterOverwrite = 1, kScatterAtomic = 2, kScatterEvec = 3, kScatterEvecAdd = 4 };
 
 template <int QF>
 struct QfTraits;
@@ -79,7 +81,8 @@ template <int P, int Q, int NCOMP, int QF, typename T>
 __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int64_t elem_end, int scatter,
                                                          const int32_t *__restrict__ offsets, int32_t comp_stride,
                                                          const T *__restrict__ q_data, const T *__restrict__ mats, const T *__restrict__ u,
-                                                         T *__restrict__ v) {
+                                                         T *__restrict__ v, T *v_l, const int32_t *__restrict__ elem_sys,
+                                                         const int32_t *__restrict__ sys_active) {
   using S = HexShape<P, Q>;
   constexpr int T1 = S::T1, NT = S::NT, SLAB = S::SLAB, P3 = S::P3, Q3 = S::Q3, QC = QfTraits<QF>::kComps;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -95,7 +98,8 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
   for (int i = threadIdx.x; i < Q * P + Q * Q; i += blockDim.x) sB[i] = mats[i];
 
   const int64_t e    = elem_begin + xcd_chunk(blockIdx.x, gridDim.x) * epb + le;
-  const bool    live = le < epb && e < elem_end;
+  // elements of systems a batched solver has frozen are skipped (elem_sys == nullptr: none)
+  const bool live = le < epb && e < elem_end && (!elem_sys || sys_active[elem_sys[e]]);
   const bool    node = live && tx < P && ty < P;
   auto sidx = [](int z, int y, int x) { return (z * T1 + y) * T1 + x; };
 
@@ -239,8 +243,14 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
         T s = 0;
 #pragma unroll
         for (int a = 0; a < Q; a++) s += sB[a * P + tx] * S1[sidx(z, ty, a)];
-        if (scatter == kScatterEvec) {  // E-vector layout [elem][comp][node]
-          v[(e * NCOMP + comp) * (int64_t)P3 + (z * P + ty) * P + tx] = s;
+        if (scatter >= kScatterEvec) {
+          if (off[z] & 1) {  // sole element of this node
+            T *dst = &v_l[(off[z] >> 1) + coff];
+            if (scatter == kScatterEvec) *dst = s;
+            else *dst += s;
+          } else {  // E-vector layout [elem][comp][node]
+            v[(e * NCOMP + comp) * (int64_t)P3 + (z * P + ty) * P + tx] = s;
+          }
           continue;
         }
         T *dst = &v[(off[z] >> 1) + coff];

@@ -20,8 +20,9 @@ NOTRANSPOSE, TRANSPOSE = 0, 1
 EVAL_NONE, EVAL_INTERP, EVAL_GRAD, EVAL_WEIGHT = 0, 1, 2, 16
 QF_MASS, QF_DIFFUSION, QF_DIFFUSION_KAPPA = 0, 1, 2
 FP64, FP32 = 0, 1
-# E^T of the fused plan: transpose offset map (default), element colours, hardware atomics
-SCATTER_FLAGS = {"tmap": 0, "color": 2, "atomic": 4}
+# E^T of the fused plan: "auto" (transpose map for P >= 5, colours below), transpose offset map,
+# element colours, hardware atomics
+SCATTER_FLAGS = {"auto": 0, "tmap": 8, "color": 2, "atomic": 4}
 GAUSS, GAUSS_LOBATTO = 0, 1
 
 
@@ -145,7 +146,7 @@ class HexOperator:
     """
 
     def __init__(self, qf: int, offsets: np.ndarray, num_elem: int, P: int, Q: int, l_size: int, num_comp: int = 1,
-                 comp_stride: int | None = None, dtype=torch.float64, quad_mode: int = GAUSS, scatter: str = "tmap"):
+                 comp_stride: int | None = None, dtype=torch.float64, quad_mode: int = GAUSS, scatter: str = "auto"):
         self.qf, self.P, self.Q, self.num_elem, self.num_comp = qf, P, Q, num_elem, num_comp
         self.dtype = dtype
         self.quad_mode = quad_mode
@@ -163,7 +164,7 @@ class HexOperator:
 
     @classmethod
     def from_mesh(cls, qf: int, mesh, Q: int | None = None, num_comp: int = 1, dtype=torch.float64, build: bool = True,
-                  scatter: str = "tmap"):
+                  scatter: str = "auto"):
         """Operator on a HexMesh with geometric factors built on the GPU from its coordinates."""
         Q = Q if Q is not None else mesh.P + 1
         op = cls(qf, mesh.offsets, mesh.num_elem, mesh.P, Q, num_comp * mesh.num_nodes, num_comp=num_comp,
