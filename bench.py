@@ -84,7 +84,10 @@ def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0)
     B = alg_bytes(m.num_nodes, m.num_elem, P, Q, s=s)
     F = alg_flops(m.num_elem, P, Q)
     res = {
-        "p": p, "n": n, "dofs": m.num_nodes, "elements": m.num_elem, "colors": len(op.colors()),
+        "p": p, "n": n, "dofs": m.num_nodes, "elements": m.num_elem,
+        "scatter": "transpose map" if P >= 5 else f"{len(op.colors())} element colours",
+        "kernels": ("one apply = hex_apply_kernel + transpose_gather_kernel (E^T by the transpose offset map)" if P >= 5
+                    else f"one apply = {len(op.colors())} hex_apply_kernel launches (one per element colour)"),
         "ms_per_apply": ms_apply, "wall_ms_per_apply": wall, "gdofs": m.num_nodes / (ms_apply * 1e-3) / 1e9,
         "kernel_ms": ms_kernel, "alg_bytes": B, "alg_flops": F,
         "kernel_GBs": B / (ms_kernel * 1e-3) / 1e9, "kernel_TFLOPs": F / (ms_kernel * 1e-3) / 1e12,
@@ -267,15 +270,16 @@ def cpu_baseline(p: int = 4, seconds: float = 10.0) -> dict | None:
             "sample": f"oracle (ref/serial restatement), p={p}, 8^3 elements, {dt:.0f} s"}
 
 
-def measured_traffic(p: int, dofs: int, dtype: str):
-    """HBM bytes per apply from the committed rocprofv3 counter summary of this workload, if one exists."""
+def measured_traffic(p: int, dofs: int, dtype: str, scatter: str):
+    """HBM bytes per apply from the committed rocprofv3 counter summary of this workload (same sizes, arithmetic
+    and E^T mode), if one exists."""
     f = ROOT / "profiles" / f"hex_apply_p{p}_8M_{dtype}_traffic.json"
     try:
         d = json.loads(f.read_text())
     except (OSError, ValueError):
         return None
     w = d.get("workload", {})
-    if w.get("p") == p and w.get("dofs") == dofs and w.get("dtype") == dtype:
+    if w.get("p") == p and w.get("dofs") == dofs and w.get("dtype") == dtype and w.get("scatter") == scatter:
         return d["traffic_bytes_per_apply"]
     return None
 
@@ -291,6 +295,8 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--dtype", choices=["f64", "f32"], default="f64",
+                    help="arithmetic of the headline apply (the metric is FP64; f32 is for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 1 GiB device copy (HBM reference for bandwidth and counter scaling)")
     args = ap.parse_args()
@@ -311,7 +317,7 @@ def main() -> None:
         print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "ms_per_copy": ms,
                           "GBs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
         return
-    head = run_case(args.p, args.dofs, torch.float64, args.steps, args.warmup)
+    head = run_case(args.p, args.dofs, torch.float64 if args.dtype == "f64" else torch.float32, args.steps, args.warmup)
     sweep = {}
     if not args.no_sweep:
         for p in range(1, 9):
@@ -345,14 +351,14 @@ def main() -> None:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": args.dtype,
         "data": "synthetic: seeded perturbed structured hex mesh, u ~ U[-1,1]",
-        "config": {"workload": f"3D Poisson operator apply, FP64, p={args.p} (P={args.p + 1}, Q={args.p + 2}), "
+        "config": {"workload": f"3D Poisson operator apply, {args.dtype.upper()}, p={args.p} (P={args.p + 1}, Q={args.p + 2}), "
                                f"{head['n']}^3 hex elements, {head['dofs']} DoFs",
-                   "p": args.p, "dofs": head["dofs"], "elements": head["elements"], "element_colors": head["colors"]},
+                   "p": args.p, "dofs": head["dofs"], "elements": head["elements"], "scatter": head["scatter"]},
         "roofline": {"bound": "hbm", "achieved": round(head["kernel_GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(head["hbm_frac"], 4), "traffic": measured_traffic(args.p, head["dofs"], "fp64"),
-                     "kernel": "hex_apply_kernel (all colour launches of one apply)",
+                     "frac": round(head["hbm_frac"], 4), "traffic": measured_traffic(args.p, head["dofs"], args.dtype, head["scatter"]),
+                     "kernel": head["kernels"],
                      "alg_bytes_per_apply": head["alg_bytes"], "kernel_ms_per_apply": round(head["kernel_ms"], 5),
                      "fp64_tflops": round(head["kernel_TFLOPs"], 2), "fp64_peak_tflops": FP64_PEAK_TFLOPS},
         "cpu_baseline": cpu,
