@@ -100,6 +100,7 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
   const int64_t e    = elem_begin + xcd_chunk(blockIdx.x, gridDim.x) * epb + le;
   // elements of systems a batched solver has frozen are skipped (elem_sys == nullptr: none)
   const bool live = le < epb && e < elem_end && (!elem_sys || sys_active[elem_sys[e]]);
+  if (elem_sys && !__syncthreads_or(live)) return;  // every element of the workgroup is frozen (uniform exit)
   const bool    node = live && tx < P && ty < P;
   auto sidx = [](int z, int y, int x) { return (z * T1 + y) * T1 + x; };
 

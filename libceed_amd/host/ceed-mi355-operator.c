@@ -64,11 +64,23 @@ static int qdata_evector(Ceed ceed, CeedElemRestriction rstr, CeedVector vec, co
 /* ------------------------------------------------------------------------ */
 /* fused path                                                                */
 /* ------------------------------------------------------------------------ */
+static bool ends_with(const char *s, const char *suffix) {
+  const size_t ls = s ? strlen(s) : 0, lx = strlen(suffix);
+  return ls >= lx && !strcmp(s + ls - lx, suffix);
+}
+
+/* Gallery pointwise operators the fused kernel inlines, recognised by kernel
+ * name and source file: this library's gallery header, or the reference
+ * gallery's files when the backend is built into a libCEED library
+ * (include/ceed/jit-source/gallery/ceed-{massapply,vectormassapply,
+ * poisson3dapply,vectorpoisson3dapply}.h). */
 static int fused_kind(const char *path, const char *name, int *qf_kind, CeedInt *num_comp) {
-  static const char gallery_file[] = "ceed-mi355-gallery.h";
-  const size_t      lp = path ? strlen(path) : 0, lg = strlen(gallery_file);
-  *qf_kind               = -1;
-  if (!path || !name || lp < lg || strcmp(path + lp - lg, gallery_file)) return 0;
+  *qf_kind = -1;
+  if (!path || !name) return 0;
+  const bool own = ends_with(path, "ceed-mi355-gallery.h");
+  if (!own && !ends_with(path, "ceed-massapply.h") && !ends_with(path, "ceed-vectormassapply.h") && !ends_with(path, "ceed-poisson3dapply.h") &&
+      !ends_with(path, "ceed-vectorpoisson3dapply.h"))
+    return 0;
   if (!strcmp(name, "MassApply")) *qf_kind = CEED_MI355_QF_MASS, *num_comp = 1;
   else if (!strcmp(name, "Vector3MassApply")) *qf_kind = CEED_MI355_QF_MASS, *num_comp = 3;
   else if (!strcmp(name, "Poisson3DApply")) *qf_kind = CEED_MI355_QF_DIFFUSION, *num_comp = 1;
