@@ -85,8 +85,8 @@ def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0)
     F = alg_flops(m.num_elem, P, Q)
     res = {
         "p": p, "n": n, "dofs": m.num_nodes, "elements": m.num_elem,
-        "scatter": "transpose map" if P >= 5 else f"{len(op.colors())} element colours",
-        "kernels": ("one apply = hex_apply_kernel + transpose_gather_kernel (E^T by the transpose offset map)" if P >= 5
+        "scatter": "transpose map" if P >= 6 else f"{len(op.colors())} element colours",
+        "kernels": ("one apply = hex_apply_kernel + transpose_gather_kernel (E^T by the transpose offset map)" if P >= 6
                     else f"one apply = {len(op.colors())} hex_apply_kernel launches (one per element colour)"),
         "ms_per_apply": ms_apply, "wall_ms_per_apply": wall, "gdofs": m.num_nodes / (ms_apply * 1e-3) / 1e9,
         "kernel_ms": ms_kernel, "alg_bytes": B, "alg_flops": F,

@@ -45,7 +45,7 @@ enum { CEED_MI355_FP64 = 0, CEED_MI355_FP32 = 1 };
  * kappa-weighted Poisson3DApply for -div(kappa grad u)). */
 enum { CEED_MI355_QF_MASS = 0, CEED_MI355_QF_DIFFUSION = 1, CEED_MI355_QF_DIFFUSION_KAPPA = 2 };
 
-/* E^T mode flags of CeedMi355OperatorCreateHex (0: transpose map for P >= 5, colours below). */
+/* E^T mode flags of CeedMi355OperatorCreateHex (0: transpose map for P >= 6, colours below). */
 enum { CEED_MI355_PLAN_COLOR = 2, CEED_MI355_PLAN_ATOMIC = 4, CEED_MI355_PLAN_TMAP = 8 };
 
 /* Geometric-factor builders (gallery Mass3DBuild / Poisson3DBuild). */
@@ -107,7 +107,7 @@ int CeedMi355BasisDestroy(CeedMi355Basis *basis);
 typedef struct CeedMi355Operator_private *CeedMi355Operator;
 
 /* Plan for a 3D tensor hex operator with a gallery pointwise D, applied by
- * one fused kernel launch.  E^T (flags; 0 picks TMAP for P_1d >= 5, COLOR below):
+ * one fused kernel launch.  E^T (flags; 0 picks TMAP for P_1d >= 6, COLOR below):
  *   CEED_MI355_PLAN_TMAP   the fused kernel writes entries reached by one
  *                          element straight to v and stores the others in
  *                          an E-vector; an ordered gather-sum over the

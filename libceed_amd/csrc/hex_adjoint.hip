@@ -26,6 +26,7 @@ __global__ void __launch_bounds__(256) hex_adjoint_kernel(int64_t elem_begin, in
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double *sB = reinterpret_cast<double *>(lds_raw);
   double *sD = sB + Q * P;
+  const double *gB = mats, *gD = mats + Q * P;  // z-direction entries: wave-uniform scalar loads
 
   const int epb = blockDim.x / NT;
   const int le = threadIdx.x / NT, t = threadIdx.x - le * NT;
@@ -82,7 +83,7 @@ __global__ void __launch_bounds__(256) hex_adjoint_kernel(int64_t elem_begin, in
     for (int qz = 0; qz < Q; qz++) {
       double s = 0;
 #pragma unroll
-      for (int b = 0; b < P; b++) s += sB[qz * P + b] * acc[b];
+      for (int b = 0; b < P; b++) s += gB[qz * P + b] * acc[b];
       xq[qz] = s;
     }
     __syncthreads();  // S0 is rewritten next
@@ -97,7 +98,7 @@ __global__ void __launch_bounds__(256) hex_adjoint_kernel(int64_t elem_begin, in
     for (int c = 0; c < Q; c++) {
       dx += sD[tx * Q + c] * S0[sidx(qz, ty, c)];
       dy += sD[ty * Q + c] * S0[sidx(qz, c, tx)];
-      dz += sD[qz * Q + c] * xq[c];
+      dz += gD[qz * Q + c] * xq[c];
     }
   };
 
@@ -138,7 +139,7 @@ __global__ void __launch_bounds__(256) hex_adjoint_kernel(int64_t elem_begin, in
     for (int a = 0; a < Q; a++) {
       s += sD[a * Q + tx] * S1[sidx(qz, ty, a)];
       s += sD[a * Q + ty] * S2[sidx(qz, a, tx)];
-      s += sD[a * Q + qz] * vz[a];
+      s += gD[a * Q + qz] * vz[a];
     }
     acc[qz] = s;
   }
@@ -147,7 +148,7 @@ __global__ void __launch_bounds__(256) hex_adjoint_kernel(int64_t elem_begin, in
   for (int b = 0; b < P; b++) {
     double s = 0;
 #pragma unroll
-    for (int qz = 0; qz < Q; qz++) s += sB[qz * P + b] * acc[qz];
+    for (int qz = 0; qz < Q; qz++) s += gB[qz * P + b] * acc[qz];
     t2[b] = s;
   }
   __syncthreads();

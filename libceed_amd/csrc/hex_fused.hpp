@@ -88,6 +88,9 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *sB = reinterpret_cast<T *>(lds_raw);
   T *sD = sB + Q * P;
+  // uniform (z-direction) matrix entries are read from global memory with
+  // wave-uniform addresses: scalar loads into SGPRs, no LDS traffic
+  const T *gB = mats, *gD = mats + Q * P;
 
   const int epb = blockDim.x / NT;
   const int le = threadIdx.x / NT, t = threadIdx.x - le * NT;
@@ -151,7 +154,7 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
       for (int qz = 0; qz < Q; qz++) {
         T s = 0;
 #pragma unroll
-        for (int b = 0; b < P; b++) s += sB[qz * P + b] * acc[b];
+        for (int b = 0; b < P; b++) s += gB[qz * P + b] * acc[b];
         uq[qz] = s;
       }
     }
@@ -164,7 +167,7 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
       for (int b = 0; b < P; b++) {
         T s = 0;
 #pragma unroll
-        for (int qz = 0; qz < Q; qz++) s += sB[qz * P + b] * uq[qz];
+        for (int qz = 0; qz < Q; qz++) s += gB[qz * P + b] * uq[qz];
         t2[b] = s;
       }
     } else {
@@ -180,7 +183,7 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
         for (int c = 0; c < Q; c++) {
           dx += sD[tx * Q + c] * S0[sidx(qz, ty, c)];
           dy += sD[ty * Q + c] * S0[sidx(qz, c, tx)];
-          dz += sD[qz * Q + c] * uq[c];
+          dz += gD[qz * Q + c] * uq[c];
         }
         T vx = 0, vy = 0, vzz = 0;
         if (live) {
@@ -207,7 +210,7 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
         for (int a = 0; a < Q; a++) {
           s += sD[a * Q + tx] * S1[sidx(qz, ty, a)];
           s += sD[a * Q + ty] * S2[sidx(qz, a, tx)];
-          s += sD[a * Q + qz] * vz[a];
+          s += gD[a * Q + qz] * vz[a];
         }
         acc[qz] = s;
       }
@@ -215,7 +218,7 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
       for (int b = 0; b < P; b++) {
         T s = 0;
 #pragma unroll
-        for (int qz = 0; qz < Q; qz++) s += sB[qz * P + b] * acc[qz];
+        for (int qz = 0; qz < Q; qz++) s += gB[qz * P + b] * acc[qz];
         t2[b] = s;
       }
     }

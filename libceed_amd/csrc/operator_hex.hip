@@ -327,12 +327,12 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
   op->comp_stride = comp_stride;
   op->l_size = l_size;
   op->apply = fn;
-  // E^T mode; by default the transpose map from P = 5 (its E-vector traffic,
-  // (P/p)^3 entries per node, pays off there) and colours below
+  // E^T mode; by default the transpose map from P = 6 (its E-vector traffic,
+  // (P/p)^3 entries per node, pays off there; measured on MI355X) and colours below
   if (flags & CEED_MI355_PLAN_ATOMIC) op->scatter = CEED_MI355_PLAN_ATOMIC;
   else if (flags & CEED_MI355_PLAN_COLOR) op->scatter = CEED_MI355_PLAN_COLOR;
   else if (flags & CEED_MI355_PLAN_TMAP) op->scatter = 0;
-  else op->scatter = P_1d >= 5 ? 0 : CEED_MI355_PLAN_COLOR;
+  else op->scatter = P_1d >= 6 ? 0 : CEED_MI355_PLAN_COLOR;
   op->epb     = mi355::default_epb(P_1d, Q_1d, qf, op->scalar_bytes);
   {
     const int tuned = tuned_epb(op);

@@ -20,7 +20,7 @@ NOTRANSPOSE, TRANSPOSE = 0, 1
 EVAL_NONE, EVAL_INTERP, EVAL_GRAD, EVAL_WEIGHT = 0, 1, 2, 16
 QF_MASS, QF_DIFFUSION, QF_DIFFUSION_KAPPA = 0, 1, 2
 FP64, FP32 = 0, 1
-# E^T of the fused plan: "auto" (transpose map for P >= 5, colours below), transpose offset map,
+# E^T of the fused plan: "auto" (transpose map for P >= 6, colours below), transpose offset map,
 # element colours, hardware atomics
 SCATTER_FLAGS = {"auto": 0, "tmap": 8, "color": 2, "atomic": 4}
 GAUSS, GAUSS_LOBATTO = 0, 1
