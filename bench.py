@@ -65,7 +65,7 @@ def time_gpu(fn, steps: int, warmup: int):
     return s.elapsed_time(e) / steps, wall
 
 
-def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0):
+def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0, scatter: str = "auto"):
     import torch
 
     from libceed_amd import ops
@@ -74,7 +74,7 @@ def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0)
     n = mesh_size_for_dofs(p, dofs)
     m = hex_mesh(n, p, seed=seed)
     P, Q = p + 1, p + 2
-    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q, dtype=dtype)
+    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q, dtype=dtype, scatter=scatter)
     g = torch.Generator(device="cuda").manual_seed(seed)
     u = (torch.rand(m.num_nodes, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dtype)
     v = torch.zeros_like(u)
@@ -82,12 +82,15 @@ def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0)
     ms_kernel, _ = time_gpu(lambda: op.apply_add(u, v), steps, warmup)
     s = 8 if dtype == torch.float64 else 4
     B = alg_bytes(m.num_nodes, m.num_elem, P, Q, s=s)
+    mode = {"tmap": "transpose map", "color": "element colours", "atomic": "atomics"}.get(
+        scatter, "transpose map" if P >= 6 else "element colours")
     F = alg_flops(m.num_elem, P, Q)
     res = {
         "p": p, "n": n, "dofs": m.num_nodes, "elements": m.num_elem,
-        "scatter": "transpose map" if P >= 6 else f"{len(op.colors())} element colours",
-        "kernels": ("one apply = hex_apply_kernel + transpose_gather_kernel (E^T by the transpose offset map)" if P >= 6
-                    else f"one apply = {len(op.colors())} hex_apply_kernel launches (one per element colour)"),
+        "scatter": mode,
+        "kernels": {"transpose map": "one apply = hex_apply_kernel + transpose_gather_kernel (E^T by the transpose offset map)",
+                    "atomics": "one apply = zero fill + hex_apply_kernel (E^T by atomic adds)"}.get(
+                        mode, f"one apply = {len(op.colors())} hex_apply_kernel launches (one per element colour)"),
         "ms_per_apply": ms_apply, "wall_ms_per_apply": wall, "gdofs": m.num_nodes / (ms_apply * 1e-3) / 1e9,
         "kernel_ms": ms_kernel, "alg_bytes": B, "alg_flops": F,
         "kernel_GBs": B / (ms_kernel * 1e-3) / 1e9, "kernel_TFLOPs": F / (ms_kernel * 1e-3) / 1e12,
@@ -295,6 +298,8 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--scatter", choices=["auto", "tmap", "color", "atomic"], default="auto",
+                    help="E^T mode of the headline plan (auto is the library default; the others are for profiling)")
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64",
                     help="arithmetic of the headline apply (the metric is FP64; f32 is for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -317,7 +322,7 @@ def main() -> None:
         print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "ms_per_copy": ms,
                           "GBs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
         return
-    head = run_case(args.p, args.dofs, torch.float64 if args.dtype == "f64" else torch.float32, args.steps, args.warmup)
+    head = run_case(args.p, args.dofs, torch.float64 if args.dtype == "f64" else torch.float32, args.steps, args.warmup, scatter=args.scatter)
     sweep = {}
     if not args.no_sweep:
         for p in range(1, 9):

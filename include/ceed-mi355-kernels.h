@@ -108,13 +108,12 @@ typedef struct CeedMi355Operator_private *CeedMi355Operator;
 
 /* Plan for a 3D tensor hex operator with a gallery pointwise D, applied by
  * one fused kernel launch.  E^T (flags; 0 picks TMAP for P_1d >= 6, COLOR below):
- *   CEED_MI355_PLAN_TMAP   the fused kernel writes entries reached by one
- *                          element straight to v and stores the others in
- *                          an E-vector; an ordered gather-sum over the
- *                          transpose offset map adds each shared L entry's
- *                          contributions in ascending (elem, comp, node)
- *                          order: no atomics, bitwise reproducible, the
- *                          reference's summation order
+ *   CEED_MI355_PLAN_TMAP   the fused kernel stores every contribution into
+ *                          an E-vector sorted by L entry (slots from the
+ *                          transpose offset map); a gather kernel sums each
+ *                          L entry's contiguous run in ascending
+ *                          (elem, comp, node) order: no atomics, bitwise
+ *                          reproducible, the reference's summation order
  *                          (ceed-ref-restriction.c:220-242);
  *   CEED_MI355_PLAN_COLOR  elements coloured so that no two of a colour share
  *                          an L entry; one launch per colour adds in place

@@ -19,7 +19,7 @@ template <int P, int Q>
 __global__ void __launch_bounds__(256) hex_adjoint_kernel(int64_t elem_begin, int64_t elem_end, int scatter, const int32_t *__restrict__ offsets,
                                                           const int32_t *__restrict__ perm, const double *__restrict__ q_data,
                                                           const double *__restrict__ mats, const double *__restrict__ u,
-                                                          const double *__restrict__ lam, double *__restrict__ v, double *v_l,
+                                                          const double *__restrict__ lam, double *__restrict__ v, const int32_t *__restrict__ tpos,
                                                           double *__restrict__ g) {
   using S = HexShape<P, Q>;
   constexpr int T1 = S::T1, NT = S::NT, SLAB = S::SLAB, P3 = S::P3, Q3 = S::Q3, QC = 7;
@@ -175,9 +175,8 @@ __global__ void __launch_bounds__(256) hex_adjoint_kernel(int64_t elem_begin, in
       double s = 0;
 #pragma unroll
       for (int a = 0; a < Q; a++) s += sB[a * P + tx] * S1[sidx(z, ty, a)];
-      if (scatter == kScatterEvec) {
-        if (off[z] & 1) v_l[off[z] >> 1] = s;  // sole element of this node
-        else v[e * (int64_t)P3 + (z * P + ty) * P + tx] = s;
+      if (scatter == kScatterEvec) {  // L-entry-sorted E-vector
+        v[tpos[e * (int64_t)P3 + (z * P + ty) * P + tx]] = s;
         continue;
       }
       double *dst = &v[off[z] >> 1];
@@ -204,7 +203,7 @@ int launch_adjoint(const CeedMi355Operator_private *op, const double *u, const d
     const int64_t grid    = (e - b + epb - 1) / epb;
     const int     scatter = tmap ? kScatterEvec : (op->scatter == CEED_MI355_PLAN_ATOMIC ? kScatterAtomic : kScatterOverwrite);
     hex_adjoint_kernel<P, Q><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, scatter, op->d_offsets, op->d_perm, (const double *)op->d_qdata,
-                                                                  (const double *)op->d_mats, u, lam, tmap ? (double *)op->d_evec : v, v, g);
+                                                                  (const double *)op->d_mats, u, lam, tmap ? (double *)op->d_evec : v, op->d_tpos, g);
   }
   MI355_LAUNCH_CHECK();
   if (tmap) return transpose_gather(op, v, 1, s);

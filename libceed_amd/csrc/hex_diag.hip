@@ -18,7 +18,7 @@ namespace {
 template <int P, int Q, typename T>
 __global__ void __launch_bounds__(128) hex_diag_kernel(int64_t elem_begin, int64_t elem_end, int scatter, int qf, int num_comp,
                                                        const int32_t *__restrict__ offsets, int32_t comp_stride, const T *__restrict__ q_data,
-                                                       int q_comp, const T *__restrict__ mats, T *__restrict__ v, T *v_l) {
+                                                       int q_comp, const T *__restrict__ mats, T *__restrict__ v, const int32_t *__restrict__ tpos) {
   constexpr int P3 = P * P * P, Q3 = Q * Q * Q;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *sB  = reinterpret_cast<T *>(lds_raw);  // Q x P
@@ -61,14 +61,8 @@ __global__ void __launch_bounds__(128) hex_diag_kernel(int64_t elem_begin, int64
     }
     const int32_t o = offsets[e * P3 + i];
     for (int k = 0; k < num_comp; k++) {
-      if (scatter >= kScatterEvec) {
-        if (o & 1) {  // sole element of this node
-          T *dst = &v_l[(o >> 1) + (int64_t)k * comp_stride];
-          if (scatter == kScatterEvec) *dst = acc;
-          else *dst += acc;
-        } else {
-          v[(e * num_comp + k) * (int64_t)P3 + i] = acc;
-        }
+      if (scatter == kScatterEvec) {  // L-entry-sorted E-vector
+        v[tpos[(e * num_comp + k) * (int64_t)P3 + i]] = acc;
         continue;
       }
       T *dst = &v[(o >> 1) + (int64_t)k * comp_stride];
@@ -95,11 +89,11 @@ int launch_diag(const CeedMi355Operator_private *op, void *v, int add, hipStream
   for (size_t c = 0; c + 1 < op->color_start.size(); c++) {
     const int64_t b = op->color_start[c], e = op->color_start[c + 1];
     if (e <= b) continue;
-    const int scatter = tmap ? (add ? kScatterEvecAdd : kScatterEvec)
+    const int scatter = tmap ? kScatterEvec
                              : (op->scatter == CEED_MI355_PLAN_ATOMIC ? kScatterAtomic : (add ? kScatterAdd : kScatterOverwrite));
     hex_diag_kernel<P, Q, T><<<(unsigned)(e - b), 128, lds, s>>>(b, e, scatter, op->qf, op->num_comp, op->d_offsets, op->comp_stride,
                                                                (const T *)op->d_qdata, op->q_comp, (const T *)op->d_mats,
-                                                               tmap ? (T *)op->d_evec : (T *)v, (T *)v);
+                                                               tmap ? (T *)op->d_evec : (T *)v, op->d_tpos);
   }
   MI355_LAUNCH_CHECK();
   if (tmap) return transpose_gather(op, v, !add, s);

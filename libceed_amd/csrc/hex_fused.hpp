@@ -24,11 +24,9 @@ namespace mi355 {
 enum { kQfMass = 0, kQfDiffusion = 1, kQfDiffusionKappa = 2 };
 // E^T modes: colour-exclusive read-modify-write (ApplyAdd), the same with the
 // first-touch overwrite (Apply), hardware atomics (one launch, any order), or
-// store of the element's E-vector block (one launch; a separate ordered
-// gather-sum over the transpose offset map completes E^T).  In the E-vector
-// modes, entries whose node no other element touches (offset bit 0 set) go
-// straight to v (= in Apply, += in ApplyAdd) and skip the E-vector.
-enum { kScatterAdd = 0, kScatterOverwrite = 1, kScatterAtomic = 2, kScatterEvec = 3, kScatterEvecAdd = 4 };
+// store of each contribution into an E-vector sorted by L entry (one launch;
+// a separate gather sums each L entry's contiguous run of contributions).
+enum { kScatterAdd = 0, kScatterOverwrite = 1, kScatterAtomic = 2, kScatterEvec = 3 };
 
 template <int QF>
 struct QfTraits;
@@ -81,7 +79,7 @@ template <int P, int Q, int NCOMP, int QF, typename T>
 __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int64_t elem_end, int scatter,
                                                          const int32_t *__restrict__ offsets, int32_t comp_stride,
                                                          const T *__restrict__ q_data, const T *__restrict__ mats, const T *__restrict__ u,
-                                                         T *__restrict__ v, T *v_l, const int32_t *__restrict__ elem_sys,
+                                                         T *__restrict__ v, const int32_t *__restrict__ tpos, const int32_t *__restrict__ elem_sys,
                                                          const int32_t *__restrict__ sys_active) {
   using S = HexShape<P, Q>;
   constexpr int T1 = S::T1, NT = S::NT, SLAB = S::SLAB, P3 = S::P3, Q3 = S::Q3, QC = QfTraits<QF>::kComps;
@@ -247,14 +245,8 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
         T s = 0;
 #pragma unroll
         for (int a = 0; a < Q; a++) s += sB[a * P + tx] * S1[sidx(z, ty, a)];
-        if (scatter >= kScatterEvec) {
-          if (off[z] & 1) {  // sole element of this node
-            T *dst = &v_l[(off[z] >> 1) + coff];
-            if (scatter == kScatterEvec) *dst = s;
-            else *dst += s;
-          } else {  // E-vector layout [elem][comp][node]
-            v[(e * NCOMP + comp) * (int64_t)P3 + (z * P + ty) * P + tx] = s;
-          }
+        if (scatter == kScatterEvec) {  // v is the L-entry-sorted E-vector; tpos gives each contribution's slot
+          v[tpos[(e * NCOMP + comp) * (int64_t)P3 + (z * P + ty) * P + tx]] = s;
           continue;
         }
         T *dst = &v[(off[z] >> 1) + coff];

@@ -33,12 +33,12 @@ struct CeedMi355Operator_private {
   int     epb;
   bool    covers_all = false;  // every L entry is reached by some element: Apply needs no zero fill
   int     scatter    = 0;      // CEED_MI355_PLAN_* E^T mode: 0 transpose map, COLOR, ATOMIC
-  // transpose offset map (scatter == 0) over the L entries not owned by a
-  // single element: entry d_tlist[j] sums the E-vector positions
-  // d_tcol[d_trow[j] .. d_trow[j+1]) in ascending (elem, comp, node) order
-  int64_t  num_targets = 0;
-  int32_t *d_tlist = nullptr, *d_trow = nullptr, *d_tcol = nullptr;
-  void    *d_evec = nullptr;  // [elem][comp][P^3] element contributions, plan scalar type
+  // transpose offset map (scatter == 0): contribution (elem, comp, node) is
+  // stored at d_evec[d_tpos[(elem * num_comp + comp) * P^3 + node]], so that
+  // L entry t owns the contiguous run d_evec[d_trow[t] .. d_trow[t+1]), in
+  // ascending (elem, comp, node) order
+  int32_t *d_trow = nullptr, *d_tpos = nullptr;
+  void    *d_evec = nullptr;  // E-vector sorted by L entry, plan scalar type
   // set by a batched solver: system of each (packed) element, and the
   // solver's per-system active flags; elements of inactive systems are skipped
   const int32_t *d_elem_sys = nullptr, *d_sys_active = nullptr;
@@ -64,11 +64,11 @@ int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v
     const int64_t b = op->color_start[c], e = op->color_start[c + 1];
     if (e <= b) continue;
     const int64_t grid    = (e - b + epb - 1) / epb;
-    const int     scatter = tmap ? (overwrite ? kScatterEvec : kScatterEvecAdd)
+    const int     scatter = tmap ? kScatterEvec
                                  : (op->scatter == CEED_MI355_PLAN_ATOMIC ? kScatterAtomic : (overwrite ? kScatterOverwrite : kScatterAdd));
     hex_apply_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, scatter, op->d_offsets, op->comp_stride, (const T *)op->d_qdata,
                                                                           (const T *)op->d_mats, (const T *)u, tmap ? (T *)op->d_evec : (T *)v,
-                                                                          (T *)v, op->d_elem_sys, op->d_sys_active);
+                                                                          op->d_tpos, op->d_elem_sys, op->d_sys_active);
   }
   MI355_LAUNCH_CHECK();
   if (tmap) return transpose_gather(op, v, overwrite, s);

@@ -229,60 +229,37 @@ int64_t elem_block(const CeedMi355Operator op) {
   return (int64_t)op->q_comp * op->Q * op->Q * op->Q;
 }
 
-// Number of (element, local node) pairs reaching each node.
-std::vector<int32_t> node_multiplicity(const int32_t *offsets, int64_t num_elem, int64_t es) {
-  int64_t n_nodes = 0;
-  for (int64_t i = 0; i < num_elem * es; i++) n_nodes = std::max<int64_t>(n_nodes, (int64_t)offsets[i] + 1);
-  std::vector<int32_t> mult((size_t)n_nodes, 0);
-  for (int64_t i = 0; i < num_elem * es; i++) mult[offsets[i]]++;
-  return mult;
-}
-
-// Transpose offset map of the E-vector [elem][comp][node] (element order =
-// caller order) over the L entries not reached exactly once: for each such
-// entry, its E positions in ascending order (none for entries no element reaches).
+// Transpose offset map of the E-vector (element order = caller order): the
+// slot of every contribution (elem, comp, node) in an E-vector sorted by L
+// entry, contributions of one L entry in ascending (elem, comp, node) order.
 int build_transpose_map(CeedMi355Operator op, const int32_t *offsets) {
   const int64_t es = (int64_t)op->P * op->P * op->P, nnz = (int64_t)op->num_elem * op->num_comp * es;
   MI355_CHECK(nnz < INT32_MAX && op->l_size < INT32_MAX, CEED_MI355_ERROR_UNSUPPORTED,
               "transpose map needs fewer than 2^31 E-vector entries (have %lld); use CEED_MI355_PLAN_COLOR", (long long)nnz);
-  std::vector<int32_t> count((size_t)op->l_size, 0);
+  std::vector<int32_t> row((size_t)op->l_size + 1, 0), pos((size_t)nnz + 1);
   for (int64_t e = 0; e < op->num_elem; e++)
     for (int64_t k = 0; k < op->num_comp; k++)
-      for (int64_t i = 0; i < es; i++) count[(size_t)(offsets[e * es + i] + k * op->comp_stride)]++;
-  std::vector<int32_t> slot((size_t)op->l_size, -1), list, row(1, 0);
-  for (int64_t t = 0; t < op->l_size; t++)
-    if (count[t] != 1) {
-      slot[t] = (int32_t)list.size();
-      list.push_back((int32_t)t);
-      row.push_back(row.back() + count[t]);
-    }
-  std::vector<int32_t> cursor(row.begin(), row.end() - 1), col((size_t)row.back() + 1);
+      for (int64_t i = 0; i < es; i++) row[(size_t)(offsets[e * es + i] + k * op->comp_stride) + 1]++;
+  for (int64_t t = 0; t < op->l_size; t++) row[t + 1] += row[t];
+  std::vector<int32_t> cursor(row.begin(), row.end() - 1);
   for (int64_t e = 0; e < op->num_elem; e++)
     for (int64_t k = 0; k < op->num_comp; k++)
-      for (int64_t i = 0; i < es; i++) {
-        const int64_t t = offsets[e * es + i] + k * op->comp_stride;
-        if (slot[t] >= 0) col[cursor[slot[t]]++] = (int32_t)((e * op->num_comp + k) * es + i);
-      }
-  op->num_targets = (int64_t)list.size();
-  MI355_HIP(hipMalloc(&op->d_tlist, sizeof(int32_t) * (list.size() + 1)));
+      for (int64_t i = 0; i < es; i++) pos[(size_t)((e * op->num_comp + k) * es + i)] = cursor[offsets[e * es + i] + k * op->comp_stride]++;
   MI355_HIP(hipMalloc(&op->d_trow, sizeof(int32_t) * row.size()));
-  MI355_HIP(hipMalloc(&op->d_tcol, sizeof(int32_t) * col.size()));
+  MI355_HIP(hipMalloc(&op->d_tpos, sizeof(int32_t) * pos.size()));
   MI355_HIP(hipMalloc(&op->d_evec, (size_t)op->scalar_bytes * (nnz + 1)));
-  if (!list.empty()) MI355_HIP(hipMemcpy(op->d_tlist, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice));
   MI355_HIP(hipMemcpy(op->d_trow, row.data(), sizeof(int32_t) * row.size(), hipMemcpyHostToDevice));
-  MI355_HIP(hipMemcpy(op->d_tcol, col.data(), sizeof(int32_t) * col.size(), hipMemcpyHostToDevice));
+  MI355_HIP(hipMemcpy(op->d_tpos, pos.data(), sizeof(int32_t) * pos.size(), hipMemcpyHostToDevice));
   return CEED_MI355_SUCCESS;
 }
 
-// v[t] (= or +=) sum of evec over t's row of the transpose map, in map order
+// v[t] (= or +=) the sum of t's contiguous run of the sorted E-vector, in order
 template <typename T>
-__global__ void __launch_bounds__(256) transpose_gather_kernel(int64_t num_targets, const int32_t *__restrict__ list, const int32_t *__restrict__ row,
-                                                               const int32_t *__restrict__ col, const T *__restrict__ evec, T *__restrict__ v,
-                                                               int overwrite) {
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < num_targets; j += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t t   = list[j];
-    T             acc = overwrite ? T(0) : v[t];
-    for (int32_t m = row[j]; m < row[j + 1]; m++) acc += evec[col[m]];
+__global__ void __launch_bounds__(256) transpose_gather_kernel(int64_t l_size, const int32_t *__restrict__ row, const T *__restrict__ evec,
+                                                               T *__restrict__ v, int overwrite) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < l_size; t += (int64_t)gridDim.x * blockDim.x) {
+    T acc = overwrite ? T(0) : v[t];
+    for (int32_t m = row[t]; m < row[t + 1]; m++) acc += evec[m];
     v[t] = acc;
   }
 }
@@ -291,14 +268,10 @@ __global__ void __launch_bounds__(256) transpose_gather_kernel(int64_t num_targe
 
 namespace mi355 {
 int transpose_gather(const CeedMi355Operator_private *op, void *v, int overwrite, hipStream_t s) {
-  if (op->num_targets == 0) return CEED_MI355_SUCCESS;
-  const int nb = mi355_blocks(op->num_targets, 256, 1 << 20);
+  const int nb = mi355_blocks(op->l_size, 256, 1 << 20);
   if (op->scalar_type == CEED_MI355_FP64)
-    transpose_gather_kernel<double><<<nb, 256, 0, s>>>(op->num_targets, op->d_tlist, op->d_trow, op->d_tcol, (const double *)op->d_evec, (double *)v,
-                                                       overwrite);
-  else
-    transpose_gather_kernel<float><<<nb, 256, 0, s>>>(op->num_targets, op->d_tlist, op->d_trow, op->d_tcol, (const float *)op->d_evec, (float *)v,
-                                                      overwrite);
+    transpose_gather_kernel<double><<<nb, 256, 0, s>>>(op->l_size, op->d_trow, (const double *)op->d_evec, (double *)v, overwrite);
+  else transpose_gather_kernel<float><<<nb, 256, 0, s>>>(op->l_size, op->d_trow, (const float *)op->d_evec, (float *)v, overwrite);
   MI355_LAUNCH_CHECK();
   return CEED_MI355_SUCCESS;
 }
@@ -375,16 +348,12 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
       delete op;
       MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED, "more than 2^30 nodes");
     }
-    std::vector<char>    touched(n_nodes, 0);
-    std::vector<int32_t> mult;
-    if (op->scatter == 0) mult = node_multiplicity(offsets_host, num_elem, es);
+    std::vector<char> touched(n_nodes, 0);
     for (int64_t p = 0; p < num_elem; p++)
       for (int i = 0; i < es; i++) {
         const int32_t node = offsets_host[(int64_t)op->perm[p] * es + i];
-        // bit 0: colours - first element (in launch order) to reach the node;
-        //        transpose map - the only element reaching the node
-        const int flag     = colored ? !touched[node] : (op->scatter == 0 && mult[node] == 1);
-        packed[p * es + i] = 2 * node + flag;
+        // bit 0 (colours): first element, in launch order, to reach the node
+        packed[p * es + i] = 2 * node + (colored && !touched[node] ? 1 : 0);
         touched[node]      = 1;
       }
     std::vector<uint8_t> hit(l_size, 0);
@@ -623,9 +592,8 @@ extern "C" int CeedMi355OperatorDestroy(CeedMi355Operator *op) {
   hipFree((*op)->d_perm);
   hipFree((*op)->d_mats);
   hipFree((*op)->d_qdata);
-  hipFree((*op)->d_tlist);
   hipFree((*op)->d_trow);
-  hipFree((*op)->d_tcol);
+  hipFree((*op)->d_tpos);
   hipFree((*op)->d_evec);
   delete *op;
   *op = nullptr;
