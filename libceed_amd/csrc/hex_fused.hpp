@@ -87,8 +87,11 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
   T *sB = reinterpret_cast<T *>(lds_raw);
   T *sD = sB + Q * P;
   // uniform (z-direction) matrix entries are read from global memory with
-  // wave-uniform addresses: scalar loads into SGPRs, no LDS traffic
-  const T *gB = mats, *gD = mats + Q * P;
+  // wave-uniform addresses: scalar loads into SGPRs, no LDS traffic.  The
+  // diffusion kernels from P = 8 need more SGPRs than a wave has for them and
+  // spill to scratch, so those read the z entries from LDS like x and y.
+  constexpr bool kScalarZ = P <= 7 || QF == kQfMass;
+  const T *gB = kScalarZ ? mats : sB, *gD = kScalarZ ? mats + Q * P : sD;
 
   const int epb = blockDim.x / NT;
   const int le = threadIdx.x / NT, t = threadIdx.x - le * NT;
