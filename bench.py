@@ -78,6 +78,7 @@ def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0,
     g = torch.Generator(device="cuda").manual_seed(seed)
     u = (torch.rand(m.num_nodes, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dtype)
     v = torch.zeros_like(u)
+    epb, _ = op.autotune(5)  # the library's elements-per-workgroup tuner (does not change results)
     ms_apply, wall = time_gpu(lambda: op.apply(u, v), steps, warmup)
     ms_kernel, _ = time_gpu(lambda: op.apply_add(u, v), steps, warmup)
     s = 8 if dtype == torch.float64 else 4
@@ -87,7 +88,7 @@ def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0,
     F = alg_flops(m.num_elem, P, Q)
     res = {
         "p": p, "n": n, "dofs": m.num_nodes, "elements": m.num_elem,
-        "scatter": mode,
+        "scatter": mode, "elems_per_block": epb,
         "kernels": {"transpose map": "one apply = hex_apply_kernel + transpose_gather_kernel (E^T by the transpose offset map)",
                     "atomics": "one apply = zero fill + hex_apply_kernel (E^T by atomic adds)"}.get(
                         mode, f"one apply = {len(op.colors())} hex_apply_kernel launches (one per element colour)"),
@@ -336,7 +337,8 @@ def main() -> None:
             r64 = head if p == args.p else run_case(p, args.dofs, torch.float64, max(10, args.steps // 4), 3)
             r32 = run_case(p, args.dofs, torch.float32, max(10, args.steps // 4), 3)
             sweep[str(p)] = {"dofs": r64["dofs"], "fp64_gdofs": round(r64["gdofs"], 3), "fp64_hbm_frac": round(r64["hbm_frac"], 4),
-                             "fp32_gdofs": round(r32["gdofs"], 3), "fp32_hbm_frac": round(r32["hbm_frac"], 4)}
+                             "fp64_epb": r64["elems_per_block"], "fp32_gdofs": round(r32["gdofs"], 3),
+                             "fp32_hbm_frac": round(r32["hbm_frac"], 4), "fp32_epb": r32["elems_per_block"]}
             print(f"# p={p}: {sweep[str(p)]}", file=sys.stderr, flush=True)
     cg = {}
     if not args.no_cg:
@@ -367,7 +369,8 @@ def main() -> None:
         "data": "synthetic: seeded perturbed structured hex mesh, u ~ U[-1,1]",
         "config": {"workload": f"3D Poisson operator apply, {args.dtype.upper()}, p={args.p} (P={args.p + 1}, Q={args.p + 2}), "
                                f"{head['n']}^3 hex elements, {head['dofs']} DoFs",
-                   "p": args.p, "dofs": head["dofs"], "elements": head["elements"], "scatter": head["scatter"]},
+                   "p": args.p, "dofs": head["dofs"], "elements": head["elements"], "scatter": head["scatter"],
+                   "elems_per_block": head["elems_per_block"]},
         "roofline": {"bound": "hbm", "achieved": round(head["kernel_GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(head["hbm_frac"], 4), "traffic": measured_traffic(args.p, head["dofs"], args.dtype, head["scatter"]),
                      "kernel": head["kernels"],

@@ -68,6 +68,13 @@ __device__ __forceinline__ void poisson_point(const T *qd, int64_t stride, T dx,
   vz = dx * s4 + dy * s3 + dz * s2;
 }
 
+template <typename T>
+__device__ __forceinline__ void poisson_regs(const T *s, T dx, T dy, T dz, T &vx, T &vy, T &vz) {
+  vx = dx * s[0] + dy * s[5] + dz * s[4];
+  vy = dx * s[5] + dy * s[1] + dz * s[3];
+  vz = dx * s[4] + dy * s[3] + dz * s[2];
+}
+
 // Workgroup -> element chunk such that the chunks one XCD runs (workgroups are
 // dealt to the 8 XCDs round-robin) are contiguous in the element order.
 __device__ __forceinline__ int64_t xcd_chunk(int64_t b, int64_t grid) {
@@ -91,6 +98,11 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
   // diffusion kernels from P = 8 need more SGPRs than a wave has for them and
   // spill to scratch, so those read the z entries from LDS like x and y.
   constexpr bool kScalarZ = P <= 7 || QF == kQfMass;
+  // The column's q_data loads are issued all at once at the start (their
+  // latency overlaps the gather and the interpolation) where the registers
+  // holding them keep three or more waves per SIMD: mass kernels and diffusion
+  // up to Q = 6.  Larger diffusion kernels load each q_z plane where it is used.
+  constexpr bool kQdEarly = QF == kQfMass || Q <= 6;
   const T *gB = kScalarZ ? mats : sB, *gD = kScalarZ ? mats + Q * P : sD;
 
   const int epb = blockDim.x / NT;
@@ -114,6 +126,14 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
     for (int z = 0; z < P; z++) off[z] = offsets[e * P3 + (z * P + ty) * P + tx];
   }
   const T *qd = q_data + (live ? e : 0) * (int64_t)QC * Q3 + (ty * Q + tx);
+  // the column's geometric factors, shared by the components (dead lanes read element 0's, unused)
+  T sq[Q][QC];
+  if constexpr (kQdEarly) {
+#pragma unroll
+    for (int qz = 0; qz < Q; qz++)
+#pragma unroll
+      for (int k = 0; k < QC; k++) sq[qz][k] = __builtin_nontemporal_load(&qd[k * Q3 + qz * Q * Q]);
+  }
   __syncthreads();
 
   for (int comp = 0; comp < NCOMP; comp++) {
@@ -163,7 +183,7 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
     if constexpr (QF == kQfMass) {
       // ---- D: v = u * (w det J)
 #pragma unroll
-      for (int qz = 0; qz < Q; qz++) uq[qz] *= live ? __builtin_nontemporal_load(&qd[qz * Q * Q]) : T(0);
+      for (int qz = 0; qz < Q; qz++) uq[qz] *= live ? sq[qz][0] : T(0);
 #pragma unroll
       for (int b = 0; b < P; b++) {
         T s = 0;
@@ -187,7 +207,15 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
           dz += gD[qz * Q + c] * uq[c];
         }
         T vx = 0, vy = 0, vzz = 0;
-        if (live) {
+        if constexpr (kQdEarly) {
+          poisson_regs<T>(sq[qz], dx, dy, dz, vx, vy, vzz);
+          if constexpr (QF == kQfDiffusionKappa) {
+            vx *= sq[qz][6];
+            vy *= sq[qz][6];
+            vzz *= sq[qz][6];
+          }
+          if (!live) vx = vy = vzz = 0;
+        } else if (live) {
           const T *q = qd + qz * Q * Q;
           poisson_point<T>(q, Q3, dx, dy, dz, vx, vy, vzz);
           if constexpr (QF == kQfDiffusionKappa) {
@@ -241,21 +269,35 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
       for (int z = 0; z < P; z++) S1[sidx(z, ty, tx)] = acc[z];
     }
     __syncthreads();
-    // ---- B^T, x direction, then E^T: scatter-add (no atomics: colour-exclusive)
+    // ---- B^T, x direction, then E^T (no atomics unless asked: colour-exclusive
+    // read-modify-write, or stores into the E-vector sorted by L entry)
     if (node) {
+      T s[P];
 #pragma unroll
       for (int z = 0; z < P; z++) {
-        T s = 0;
+        T acc = 0;
 #pragma unroll
-        for (int a = 0; a < Q; a++) s += sB[a * P + tx] * S1[sidx(z, ty, a)];
-        if (scatter == kScatterEvec) {  // v is the L-entry-sorted E-vector; tpos gives each contribution's slot
-          v[tpos[(e * NCOMP + comp) * (int64_t)P3 + (z * P + ty) * P + tx]] = s;
-          continue;
-        }
-        T *dst = &v[(off[z] >> 1) + coff];
-        if (scatter == kScatterAtomic) unsafeAtomicAdd(dst, s);                        // all elements in one launch
-        else if (scatter == kScatterOverwrite && (off[z] & 1)) *dst = s;  // first element to reach this entry: CeedOperatorApply writes
-        else *dst += s;
+        for (int a = 0; a < Q; a++) acc += sB[a * P + tx] * S1[sidx(z, ty, a)];
+        s[z] = acc;
+      }
+      if (scatter == kScatterEvec) {  // tpos gives each contribution's slot in the sorted E-vector
+        int32_t pos[P];
+#pragma unroll
+        for (int z = 0; z < P; z++) pos[z] = tpos[(e * NCOMP + comp) * (int64_t)P3 + (z * P + ty) * P + tx];
+#pragma unroll
+        for (int z = 0; z < P; z++) v[pos[z]] = s[z];
+      } else if (scatter == kScatterAtomic) {  // all elements in one launch
+#pragma unroll
+        for (int z = 0; z < P; z++) unsafeAtomicAdd(&v[(off[z] >> 1) + coff], s[z]);
+      } else {
+        // all reads of v before the writes, so the loads are in flight together;
+        // Apply's first element to reach an entry (first-touch bit) writes it without reading
+        const bool overwrite = scatter == kScatterOverwrite;
+        T          old[P];
+#pragma unroll
+        for (int z = 0; z < P; z++) old[z] = (overwrite && (off[z] & 1)) ? T(0) : v[(off[z] >> 1) + coff];
+#pragma unroll
+        for (int z = 0; z < P; z++) v[(off[z] >> 1) + coff] = (overwrite && (off[z] & 1)) ? s[z] : old[z] + s[z];
       }
     }
   }
