@@ -44,7 +44,7 @@ def alg_bytes(N: int, E: int, P: int, Q: int, ncomp: int = 1, qcomp: int = 6, s:
 
 def alg_flops(E: int, P: int, Q: int, ncomp: int = 1) -> int:
     """Sum-factorised interp + collocated grad, forward and transpose, plus 15 flops/qpt of Poisson3DApply."""
-    return E * (2 * (2 * (P ** 3 * Q + P ** 2 * Q ** 2 + P * Q ** 3) + 6 * Q ** 4 * ncomp) + 15 * Q ** 3 + 2 * Q ** 3)
+    return E * ncomp * (2 * (2 * (P ** 3 * Q + P ** 2 * Q ** 2 + P * Q ** 3) + 6 * Q ** 4) + 15 * Q ** 3 + 2 * Q ** 3)
 
 
 def time_gpu(fn, steps: int, warmup: int):
@@ -65,7 +65,7 @@ def time_gpu(fn, steps: int, warmup: int):
     return s.elapsed_time(e) / steps, wall
 
 
-def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0, scatter: str = "auto"):
+def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0, scatter: str = "auto", epb: int = 0, num_comp: int = 1):
     import torch
 
     from libceed_amd import ops
@@ -74,25 +74,28 @@ def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0,
     n = mesh_size_for_dofs(p, dofs)
     m = hex_mesh(n, p, seed=seed)
     P, Q = p + 1, p + 2
-    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q, dtype=dtype, scatter=scatter)
+    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q, num_comp=num_comp, dtype=dtype, scatter=scatter)
     g = torch.Generator(device="cuda").manual_seed(seed)
-    u = (torch.rand(m.num_nodes, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dtype)
+    u = (torch.rand(num_comp * m.num_nodes, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dtype)
     v = torch.zeros_like(u)
-    epb, _ = op.autotune(5)  # the library's elements-per-workgroup tuner (does not change results)
+    if epb:
+        op.set_launch(epb)
+    else:
+        epb, _ = op.autotune(5)  # the library's elements-per-workgroup tuner (does not change results)
     ms_apply, wall = time_gpu(lambda: op.apply(u, v), steps, warmup)
     ms_kernel, _ = time_gpu(lambda: op.apply_add(u, v), steps, warmup)
     s = 8 if dtype == torch.float64 else 4
-    B = alg_bytes(m.num_nodes, m.num_elem, P, Q, s=s)
+    B = alg_bytes(m.num_nodes, m.num_elem, P, Q, ncomp=num_comp, s=s)
     mode = {"tmap": "transpose map", "color": "element colours", "atomic": "atomics"}.get(
         scatter, "transpose map" if P >= 6 else "element colours")
-    F = alg_flops(m.num_elem, P, Q)
+    F = alg_flops(m.num_elem, P, Q, ncomp=num_comp)
     res = {
-        "p": p, "n": n, "dofs": m.num_nodes, "elements": m.num_elem,
+        "p": p, "n": n, "dofs": num_comp * m.num_nodes, "num_comp": num_comp, "elements": m.num_elem,
         "scatter": mode, "elems_per_block": epb,
         "kernels": {"transpose map": "one apply = hex_apply_kernel + transpose_gather_kernel (E^T by the transpose offset map)",
                     "atomics": "one apply = zero fill + hex_apply_kernel (E^T by atomic adds)"}.get(
                         mode, f"one apply = {len(op.colors())} hex_apply_kernel launches (one per element colour)"),
-        "ms_per_apply": ms_apply, "wall_ms_per_apply": wall, "gdofs": m.num_nodes / (ms_apply * 1e-3) / 1e9,
+        "ms_per_apply": ms_apply, "wall_ms_per_apply": wall, "gdofs": num_comp * m.num_nodes / (ms_apply * 1e-3) / 1e9,
         "kernel_ms": ms_kernel, "alg_bytes": B, "alg_flops": F,
         "kernel_GBs": B / (ms_kernel * 1e-3) / 1e9, "kernel_TFLOPs": F / (ms_kernel * 1e-3) / 1e12,
     }
@@ -176,6 +179,55 @@ def run_cg_mixed(p: int, dofs: float, ks=(10, 25, 50), rtol: float = 1e-10) -> d
     out["best_k"] = int(best)
     out["speedup"] = mixed[best]["speedup"]
     return out
+
+
+def run_adjoint(dofs: float = 4e6, p: int = 4, seed: int = 0, directions: int = 64, h: float = 1e-6) -> dict:
+    """Variable-coefficient diffusion -div(kappa grad u), p=4, Q=5, FP64, kappa = exp(g) at the quadrature points with g a
+    smooth random field: forward apply, fused adjoint (A^T lambda and dL/dkappa in one kernel), and central finite
+    differences of L = 1/2 ||A(kappa) u - f||^2 along random kappa directions."""
+    import torch
+
+    from libceed_amd import ops
+    from libceed_amd.mesh import hex_mesh, mesh_size_for_dofs, quadrature_points, smooth_random_field
+
+    m = hex_mesh(mesh_size_for_dofs(p, dofs), p, seed=seed)
+    P, Q = p + 1, p + 1
+    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION_KAPPA, m, Q=Q)
+    gq = np.array(ops.gauss_quadrature(Q)[0])
+    kappa = torch.from_numpy(np.exp(smooth_random_field(quadrature_points(m, gq), seed=seed + 5))).cuda()
+    op.set_kappa(kappa)
+    rng = np.random.default_rng(seed)
+    u = torch.from_numpy(rng.uniform(-1, 1, m.num_nodes)).cuda()
+    f = torch.from_numpy(rng.uniform(-1, 1, m.num_nodes)).cuda()
+    v = torch.zeros_like(u)
+    lam = torch.zeros_like(u)
+    dk = torch.zeros_like(kappa)
+    ms_fwd, _ = time_gpu(lambda: op.apply(u, v), 20, 3)
+    op.apply(u, v)
+    r = v - f  # dL/d(Au)
+    ms_adj, _ = time_gpu(lambda: op.apply_adjoint(u, r, lam, dk), 20, 3)
+    op.apply_adjoint(u, r, lam, dk)
+
+    def loss(k):
+        op.set_kappa(k)
+        op.apply(u, v)
+        return 0.5 * float(torch.dot(v - f, v - f))
+
+    # L is quadratic in kappa (A is linear in it), so central differences are exact up to roundoff for any step:
+    # h = 1e-6 shows the roundoff of the loss divided by 2h, the larger step the agreement itself
+    worst = {h: 0.0, 1e-2: 0.0}
+    g = torch.Generator(device="cuda").manual_seed(seed + 1)
+    for _ in range(directions):
+        d = torch.rand(kappa.shape, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+        an = float(torch.dot(dk, d))
+        for step in worst:
+            fd = (loss(kappa + step * d) - loss(kappa - step * d)) / (2 * step)
+            worst[step] = max(worst[step], abs(fd - an) / max(abs(an), 1e-300))
+    op.set_kappa(kappa)
+    return {"p": p, "Q": Q, "dofs": m.num_nodes, "elements": m.num_elem, "ms_forward": round(ms_fwd, 4),
+            "ms_adjoint_v_and_dkappa": round(ms_adj, 4), "forward_gdofs": round(m.num_nodes / (ms_fwd * 1e-3) / 1e9, 3),
+            "adjoint_gdofs": round(m.num_nodes / (ms_adj * 1e-3) / 1e9, 3), "fd_directions": directions,
+            "fd_max_rel_error": {f"h={k:g}": v for k, v in worst.items()}}
 
 
 def run_batch(count: int = 256, rtol: float = 1e-8, seed: int = 0) -> dict:
@@ -310,6 +362,8 @@ def main() -> None:
                     help="E^T mode of the headline plan (auto is the library default; the others are for profiling)")
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64",
                     help="arithmetic of the headline apply (the metric is FP64; f32 is for profiling)")
+    ap.add_argument("--epb", type=int, default=0,
+                    help="elements per workgroup of the headline plan (default: autotuned; fixed values are for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 1 GiB device copy (HBM reference for bandwidth and counter scaling)")
     args = ap.parse_args()
@@ -327,10 +381,14 @@ def main() -> None:
         a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
         ms, _ = time_gpu(lambda: b.copy_(a), args.steps, args.warmup)
+        big = torch.ones(4 * nbytes // 8, dtype=torch.float64, device="cuda")
+        ms_read, _ = time_gpu(lambda: big.sum(), args.steps, args.warmup)
         print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "ms_per_copy": ms,
-                          "GBs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+                          "GBs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9,
+                          "sum_bytes_read": 4 * nbytes, "ms_per_sum": ms_read, "GBs_read": 4 * nbytes / (ms_read * 1e-3) / 1e9}), flush=True)
         return
-    head = run_case(args.p, args.dofs, torch.float64 if args.dtype == "f64" else torch.float32, args.steps, args.warmup, scatter=args.scatter)
+    head = run_case(args.p, args.dofs, torch.float64 if args.dtype == "f64" else torch.float32, args.steps, args.warmup, scatter=args.scatter,
+                    epb=args.epb)
     sweep = {}
     if not args.no_sweep:
         for p in range(1, 9):
@@ -340,6 +398,18 @@ def main() -> None:
                              "fp64_epb": r64["elems_per_block"], "fp32_gdofs": round(r32["gdofs"], 3),
                              "fp32_hbm_frac": round(r32["hbm_frac"], 4), "fp32_epb": r32["elems_per_block"]}
             print(f"# p={p}: {sweep[str(p)]}", file=sys.stderr, flush=True)
+    sizes = {}
+    if not args.no_sweep:
+        for label, d, nc in (("p4_2M", 2e6, 1), ("p4_32M", 32e6, 1), ("p4_8M_vector3", 8e6, 3)):
+            r = run_case(4, d, torch.float64, max(10, args.steps // 4), 3, num_comp=nc)
+            sizes[label] = {"dofs": r["dofs"], "num_comp": nc, "fp64_gdofs": round(r["gdofs"], 3), "fp64_hbm_frac": round(r["hbm_frac"], 4),
+                            "epb": r["elems_per_block"]}
+            print(f"# {label}: {sizes[label]}", file=sys.stderr, flush=True)
+    adjoint = None
+    if not args.no_sweep:
+        adjoint = run_adjoint()
+        print(f"# adjoint: {adjoint}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
     cg = {}
     if not args.no_cg:
         cg["fp64_200_iterations_p4_8M"] = run_cg_fixed(4, 8e6)
@@ -378,6 +448,8 @@ def main() -> None:
                      "fp64_tflops": round(head["kernel_TFLOPs"], 2), "fp64_peak_tflops": FP64_PEAK_TFLOPS},
         "cpu_baseline": cpu,
         "sweep": sweep,
+        "sizes": sizes,
+        "adjoint": adjoint,
         "cg": cg,
         "batched_solves": batch,
     }
