@@ -65,7 +65,8 @@ def time_gpu(fn, steps: int, warmup: int):
     return s.elapsed_time(e) / steps, wall
 
 
-def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0, scatter: str = "auto", epb: int = 0, num_comp: int = 1):
+def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0, scatter: str = "auto", epb: int = 0, num_comp: int = 1,
+             layout: int = 1):
     import torch
 
     from libceed_amd import ops
@@ -79,9 +80,10 @@ def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0,
     u = (torch.rand(num_comp * m.num_nodes, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dtype)
     v = torch.zeros_like(u)
     if epb:
-        op.set_launch(epb)
+        op.set_layout(layout, epb)
     else:
-        epb, _ = op.autotune(5)  # the library's elements-per-workgroup tuner (does not change results)
+        op.autotune(5)  # the library's (thread layout, elements per workgroup) tuner; results are bitwise the same for all
+    layout, epb = op.launch()
     ms_apply, wall = time_gpu(lambda: op.apply(u, v), steps, warmup)
     ms_kernel, _ = time_gpu(lambda: op.apply_add(u, v), steps, warmup)
     s = 8 if dtype == torch.float64 else 4
@@ -89,12 +91,13 @@ def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0,
     mode = {"tmap": "transpose map", "color": "element colours", "atomic": "atomics"}.get(
         scatter, "transpose map" if P >= 6 else "element colours")
     F = alg_flops(m.num_elem, P, Q, ncomp=num_comp)
+    kname = "hex_apply_lines_kernel" if layout == 1 else "hex_apply_kernel"
     res = {
         "p": p, "n": n, "dofs": num_comp * m.num_nodes, "num_comp": num_comp, "elements": m.num_elem,
-        "scatter": mode, "elems_per_block": epb,
-        "kernels": {"transpose map": "one apply = hex_apply_kernel + transpose_gather_kernel (E^T by the transpose offset map)",
-                    "atomics": "one apply = zero fill + hex_apply_kernel (E^T by atomic adds)"}.get(
-                        mode, f"one apply = {len(op.colors())} hex_apply_kernel launches (one per element colour)"),
+        "scatter": mode, "layout": "lines" if layout == 1 else "columns", "elems_per_block": epb,
+        "kernels": {"transpose map": f"one apply = {kname} + transpose_gather_kernel (E^T by the transpose offset map)",
+                    "atomics": f"one apply = zero fill + {kname} (E^T by atomic adds)"}.get(
+                        mode, f"one apply = {len(op.colors())} {kname} launches (one per element colour)"),
         "ms_per_apply": ms_apply, "wall_ms_per_apply": wall, "gdofs": num_comp * m.num_nodes / (ms_apply * 1e-3) / 1e9,
         "kernel_ms": ms_kernel, "alg_bytes": B, "alg_flops": F,
         "kernel_GBs": B / (ms_kernel * 1e-3) / 1e9, "kernel_TFLOPs": F / (ms_kernel * 1e-3) / 1e12,
@@ -261,7 +264,8 @@ def run_batch(count: int = 256, rtol: float = 1e-8, seed: int = 0) -> dict:
     lat = sec * st["converged_at_check"] / max(1, st["num_checks"])  # every check interval costs the same work
     dofs = int(bcg.n)
     out = {"systems": count, "dofs_total": dofs, "groups": len(bcg.ops), "rtol": rtol, "setup_seconds": round(setup, 2),
-           "autotuned_elems_per_block": {f"{'mass' if k[0] == 0 else 'diff'}_P{k[1]}_Q{k[2]}": v[0] for k, v in bcg.tuning.items()},
+           "autotuned_launch": {f"{'mass' if k[0] == 0 else 'diff'}_P{k[1]}_Q{k[2]}": ["lines" if v[0] == 1 else "columns", v[1]]
+                                for k, v in bcg.tuning.items()},
            "batched": {"seconds": round(sec, 4), "solves_per_s": round(count / sec, 2), "iterations_max": int(st["iterations"].max()),
                        "latency_p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
                        "latency_p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
@@ -364,6 +368,8 @@ def main() -> None:
                     help="arithmetic of the headline apply (the metric is FP64; f32 is for profiling)")
     ap.add_argument("--epb", type=int, default=0,
                     help="elements per workgroup of the headline plan (default: autotuned; fixed values are for profiling)")
+    ap.add_argument("--layout", choices=["lines", "columns"], default="lines",
+                    help="thread layout of the headline plan when --epb is given")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 1 GiB device copy (HBM reference for bandwidth and counter scaling)")
     args = ap.parse_args()
@@ -388,22 +394,22 @@ def main() -> None:
                           "sum_bytes_read": 4 * nbytes, "ms_per_sum": ms_read, "GBs_read": 4 * nbytes / (ms_read * 1e-3) / 1e9}), flush=True)
         return
     head = run_case(args.p, args.dofs, torch.float64 if args.dtype == "f64" else torch.float32, args.steps, args.warmup, scatter=args.scatter,
-                    epb=args.epb)
+                    epb=args.epb, layout=1 if args.layout == "lines" else 0)
     sweep = {}
     if not args.no_sweep:
         for p in range(1, 9):
             r64 = head if p == args.p else run_case(p, args.dofs, torch.float64, max(10, args.steps // 4), 3)
             r32 = run_case(p, args.dofs, torch.float32, max(10, args.steps // 4), 3)
             sweep[str(p)] = {"dofs": r64["dofs"], "fp64_gdofs": round(r64["gdofs"], 3), "fp64_hbm_frac": round(r64["hbm_frac"], 4),
-                             "fp64_epb": r64["elems_per_block"], "fp32_gdofs": round(r32["gdofs"], 3),
-                             "fp32_hbm_frac": round(r32["hbm_frac"], 4), "fp32_epb": r32["elems_per_block"]}
+                             "fp64_launch": [r64["layout"], r64["elems_per_block"]], "fp32_gdofs": round(r32["gdofs"], 3),
+                             "fp32_hbm_frac": round(r32["hbm_frac"], 4), "fp32_launch": [r32["layout"], r32["elems_per_block"]]}
             print(f"# p={p}: {sweep[str(p)]}", file=sys.stderr, flush=True)
     sizes = {}
     if not args.no_sweep:
         for label, d, nc in (("p4_2M", 2e6, 1), ("p4_32M", 32e6, 1), ("p4_8M_vector3", 8e6, 3)):
             r = run_case(4, d, torch.float64, max(10, args.steps // 4), 3, num_comp=nc)
             sizes[label] = {"dofs": r["dofs"], "num_comp": nc, "fp64_gdofs": round(r["gdofs"], 3), "fp64_hbm_frac": round(r["hbm_frac"], 4),
-                            "epb": r["elems_per_block"]}
+                            "launch": [r["layout"], r["elems_per_block"]]}
             print(f"# {label}: {sizes[label]}", file=sys.stderr, flush=True)
     adjoint = None
     if not args.no_sweep:
@@ -440,7 +446,7 @@ def main() -> None:
         "config": {"workload": f"3D Poisson operator apply, {args.dtype.upper()}, p={args.p} (P={args.p + 1}, Q={args.p + 2}), "
                                f"{head['n']}^3 hex elements, {head['dofs']} DoFs",
                    "p": args.p, "dofs": head["dofs"], "elements": head["elements"], "scatter": head["scatter"],
-                   "elems_per_block": head["elems_per_block"]},
+                   "layout": head["layout"], "elems_per_block": head["elems_per_block"]},
         "roofline": {"bound": "hbm", "achieved": round(head["kernel_GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(head["hbm_frac"], 4), "traffic": measured_traffic(args.p, head["dofs"], args.dtype, head["scatter"]),
                      "kernel": head["kernels"],

@@ -161,10 +161,19 @@ int CeedMi355OperatorApply(CeedMi355Operator op, const void *u, void *v, void *s
 /* Elements per workgroup of the fused kernel (1 <= epb, epb * Q^2 <= 256
  * threads, LDS <= 64 KiB). */
 int CeedMi355OperatorSetLaunch(CeedMi355Operator op, int32_t elems_per_block);
-/* Autotuner: time Apply (reps times) for every admissible elements-per-
- * workgroup value on scratch vectors, keep the fastest in the plan and
- * remember it for later plans of the same (P, Q, num_comp, pointwise
- * operator, scalar type, E^T mode). */
+/* Thread layout of the fused kernel: 0 = z-columns per thread, x and y
+ * contractions read point by point from LDS; 1 = whole 1D lines per thread
+ * for the x and y contractions (about a third of the LDS traffic).  Both give
+ * identical results.  elems_per_block <= 0 picks the largest that fits. */
+int CeedMi355OperatorSetLayout(CeedMi355Operator op, int32_t layout, int32_t elems_per_block);
+/* Current thread layout and elements per workgroup (either pointer may be NULL). */
+int CeedMi355OperatorGetLaunch(CeedMi355Operator op, int32_t *layout, int32_t *elems_per_block);
+/* Autotuner (the reference has none: its hip-shared/hip-gen backends fix the
+ * launch shape per P, Q): time Apply (reps times) for every admissible
+ * (thread layout, elements per workgroup) on scratch vectors, keep the
+ * fastest in the plan and remember it for later plans of the same (P, Q,
+ * num_comp, pointwise operator, scalar type, E^T mode).  best_epb_host gets
+ * the elements per workgroup; CeedMi355OperatorGetLaunch gives the layout. */
 int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int32_t *best_epb_host, double *best_ms_host, void *stream);
 /* Number of element colours, and the element count of each (host array of at
  * least num_colors entries, may be NULL). */

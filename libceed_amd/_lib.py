@@ -58,6 +58,8 @@ SIGNATURES = {
     "CeedMi355BatchCGDestroy": (c_int, [C.POINTER(vp)]),
     "CeedMi355OperatorSetLaunch": (c_int, [vp, c_i32]),
     "CeedMi355OperatorAutotune": (c_int, [vp, c_i32, vp, vp, vp]),
+    "CeedMi355OperatorSetLayout": (c_int, [vp, c_i32, c_i32]),
+    "CeedMi355OperatorGetLaunch": (c_int, [vp, vp, vp]),
     "CeedRegister_Mi355": (c_int, []),
     "CeedInit_Mi355": (c_int, [C.c_char_p, vp]),
 }

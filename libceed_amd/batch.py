@@ -40,7 +40,7 @@ def random_systems(count: int = 256, seed: int = 0, max_n: int = 24, p_range=(2,
 
 
 class BatchCG:
-    """Batched CG over a list of systems; autotune=True tunes each group's elements per workgroup."""
+    """Batched CG over a list of systems; autotune=True tunes each group's thread layout and elements per workgroup."""
 
     def __init__(self, systems: list[System], scatter: str = "auto", autotune: bool = True):
         self.systems = systems
@@ -49,7 +49,7 @@ class BatchCG:
             groups.setdefault((s.kind, s.mesh.P, s.mesh.P + 1), []).append(i)
         self.order: list[int] = []          # batch position -> system index
         self.ops: list[ops.HexOperator] = []
-        self.tuning: dict[tuple, tuple[int, float]] = {}
+        self.tuning: dict[tuple, tuple[int, int, float]] = {}  # (kind, P, Q) -> (thread layout, elements per workgroup, ms)
         op_begin, sys_begin = [], [0]
         at = 0
         for (kind, P, Q), members in sorted(groups.items()):
@@ -71,9 +71,9 @@ class BatchCG:
             xyz = torch.from_numpy(np.concatenate([np.concatenate(c) for c in coords])).cuda()
             op.build_qdata(np.concatenate(moffs), xyz, vert_shift, mesh_P=2)
             if autotune:
-                best, ms = C.c_int32(), C.c_double()
-                _lib.call("CeedMi355OperatorAutotune", op._h, 5, C.byref(best), C.byref(ms), ops._stream())
-                self.tuning[(kind, P, Q)] = (best.value, ms.value)
+                _, ms = op.autotune(5)
+                layout, epb = op.launch()
+                self.tuning[(kind, P, Q)] = (layout, epb, ms)
             self.ops.append(op)
             op_begin.append(at)
             at += node_shift

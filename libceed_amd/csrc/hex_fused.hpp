@@ -57,22 +57,28 @@ __host__ __device__ constexpr int hex_lds_entries(int epb) {
 
 // Pointwise Poisson3DApply (jit-source/gallery/ceed-poisson3dapply.h):
 // vg_j = sum_i ug_i S_ij with S in Voigt order {xx, yy, zz, yz, xz, xy}.
-// q_data is streamed once per apply: non-temporal loads keep it from displacing u and v in the caches.
-template <typename T>
-__device__ __forceinline__ void poisson_point(const T *qd, int64_t stride, T dx, T dy, T dz, T &vx, T &vy, T &vz) {
-  const T s0 = __builtin_nontemporal_load(qd), s1 = __builtin_nontemporal_load(qd + stride), s2 = __builtin_nontemporal_load(qd + 2 * stride),
-          s3 = __builtin_nontemporal_load(qd + 3 * stride), s4 = __builtin_nontemporal_load(qd + 4 * stride),
-          s5 = __builtin_nontemporal_load(qd + 5 * stride);
-  vx = dx * s0 + dy * s5 + dz * s4;
-  vy = dx * s5 + dy * s1 + dz * s3;
-  vz = dx * s4 + dy * s3 + dz * s2;
-}
+// The multiply-adds are written out as fmas so that every kernel computing
+// it (both thread layouts, with q_data from registers or loaded in place)
+// rounds identically, whatever contraction the compiler would choose.
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float  fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 template <typename T>
 __device__ __forceinline__ void poisson_regs(const T *s, T dx, T dy, T dz, T &vx, T &vy, T &vz) {
-  vx = dx * s[0] + dy * s[5] + dz * s[4];
-  vy = dx * s[5] + dy * s[1] + dz * s[3];
-  vz = dx * s[4] + dy * s[3] + dz * s[2];
+  vx = fma_t(dz, s[4], fma_t(dy, s[5], dx * s[0]));
+  vy = fma_t(dz, s[3], fma_t(dy, s[1], dx * s[5]));
+  vz = fma_t(dz, s[2], fma_t(dy, s[3], dx * s[4]));
+}
+
+// The same with the six entries loaded in place (stride between entries);
+// q_data is streamed once per apply: non-temporal loads keep it from
+// displacing u and v in the caches.
+template <typename T>
+__device__ __forceinline__ void poisson_point(const T *qd, int64_t stride, T dx, T dy, T dz, T &vx, T &vy, T &vz) {
+  T s[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) s[k] = __builtin_nontemporal_load(qd + k * stride);
+  poisson_regs<T>(s, dx, dy, dz, vx, vy, vz);
 }
 
 // Workgroup -> element chunk such that the chunks one XCD runs (workgroups are
@@ -231,16 +237,19 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
       }
       __syncthreads();
       // ---- G^T: sum of the three direction transposes, then B^T in z
+      // (x and y sums first, then the z terms: the order the line layout uses)
       T acc[Q];
 #pragma unroll
       for (int qz = 0; qz < Q; qz++) {
-        T s = 0;
+        T sx = 0, sy = 0;
 #pragma unroll
         for (int a = 0; a < Q; a++) {
-          s += sD[a * Q + tx] * S1[sidx(qz, ty, a)];
-          s += sD[a * Q + ty] * S2[sidx(qz, a, tx)];
-          s += gD[a * Q + qz] * vz[a];
+          sx += sD[a * Q + tx] * S1[sidx(qz, ty, a)];
+          sy += sD[a * Q + ty] * S2[sidx(qz, a, tx)];
         }
+        T s = sx + sy;
+#pragma unroll
+        for (int a = 0; a < Q; a++) s += gD[a * Q + qz] * vz[a];
         acc[qz] = s;
       }
 #pragma unroll

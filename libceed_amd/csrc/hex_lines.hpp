@@ -1,0 +1,340 @@
+// Fused per-element hex operator, "line" thread layout.
+//
+// Same operator as hex_apply_kernel (gather E, sum-factorised interp and
+// collocated gradient B, pointwise D, transposes, E^T), with a different
+// assignment of work to threads.  The column-slab kernel gives thread (x, y)
+// the z-column of its point and contracts x and y through LDS, so every
+// output point reads Q values back from LDS (Q-fold redundant reads of the
+// same slab row by the Q threads of that row).  Here, for each x- or
+// y-contraction, a thread owns one whole 1D line of the slab: it reads the
+// line's Q (or P) inputs once, computes all of the line's outputs with the
+// 1D matrix entries (wave-uniform: scalar loads, no LDS), and writes them
+// back.  Per element and component this reads and writes each slab entry
+// about once per contraction instead of Q times, so the LDS traffic, which
+// bounds the column-slab kernel at p >= 3, drops about threefold.  The
+// z-contractions and the pointwise operator stay in registers on the
+// (x, y) columns, as in the column-slab kernel.
+//
+// Thread t of an element's Q*Q threads plays these roles:
+//   column (qx, qy) = (t % Q, t / Q)                     z-contractions, pointwise D
+//   node column (x, y) = (t % Q, t / Q), x, y < P      gather E, scatter E^T (coalesced)
+//   x-line of nodes (y, z) = (t % P, t / P), t < P*P     B_x, B_x^T
+//   y-line (qx, z) = (t % Q, t / Q), t < Q*P              B_y, B_y^T
+//   x-line (qy, qz) and y-line (qx, qz) = (t % Q, t / Q)  the collocated derivatives
+// Slabs are [z][y][x] with strides (Q*Q, Q, 1); mass needs two per element,
+// diffusion four (the x and y derivative transposes land in separate slabs).
+#pragma once
+
+#include "hex_fused.hpp"
+
+namespace mi355 {
+
+// The 1D matrices are read with wave-uniform addresses (scalar loads into
+// SGPRs).  Loaded once for the whole kernel they would need ~2(QP + Q^2)
+// SGPRs, far more than a wave has, and the compiler would spill them to VGPR
+// lanes (a v_writelane/v_readlane pair per use).  Re-deriving the pointer
+// through an empty asm at the start of each phase keeps the loads inside the
+// phase that uses them, so only that phase's entries are live.
+// The pointer is returned in the constant address space (read-only for the
+// kernel's lifetime, which the matrices are), so the loads stay scalar.
+template <typename T>
+using const_ptr = const __attribute__((address_space(4))) T *;
+
+template <typename T>
+__device__ __forceinline__ const_ptr<T> phase_ptr(const T *p) {
+  asm volatile("" : "+s"(p));
+  return (const_ptr<T>)p;
+}
+
+template <int QF>
+struct LineSlabs {
+  static constexpr int kSlabs = QF == kQfMass ? 2 : 4;
+};
+
+template <int P, int Q, int QF>
+__host__ __device__ constexpr int hex_lines_lds_entries(int epb) {
+  return epb * LineSlabs<QF>::kSlabs * Q * Q * Q;
+}
+
+template <int P, int Q, int NCOMP, int QF, typename T>
+__global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin, int64_t elem_end, int scatter,
+                                                               const int32_t *__restrict__ offsets, int32_t comp_stride,
+                                                               const T *__restrict__ q_data, const T *__restrict__ mats,
+                                                               const T *__restrict__ u, T *__restrict__ v, const int32_t *__restrict__ tpos,
+                                                               const int32_t *__restrict__ elem_sys, const int32_t *__restrict__ sys_active) {
+  constexpr int NT = Q * Q, SLAB = Q * Q * Q, P3 = P * P * P, Q3 = Q * Q * Q, QC = QfTraits<QF>::kComps;
+  constexpr int NS = LineSlabs<QF>::kSlabs;
+  // from Q = 7 one phase's matrix entries no longer fit the SGPRs: load them
+  // per output row (or column) instead of per phase
+  constexpr bool kRowWise = Q >= 7;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  // B (Q x P) then D (Q x Q), row-major, at mats
+
+  const int epb = blockDim.x / NT;
+  const int le = threadIdx.x / NT, t = threadIdx.x - le * NT;
+  T        *S0 = reinterpret_cast<T *>(lds_raw) + (size_t)le * NS * SLAB;
+  T        *S1 = S0 + SLAB;
+  T        *S2 = S1 + SLAB;  // diffusion only
+  T        *S3 = S2 + SLAB;  // diffusion only
+  auto      at = [](int z, int y, int x) { return (z * Q + y) * Q + x; };
+
+  const int64_t e    = elem_begin + xcd_chunk(blockIdx.x, gridDim.x) * epb + le;
+  const bool    live = le < epb && e < elem_end && (!elem_sys || sys_active[elem_sys[e]]);
+  if (elem_sys && !__syncthreads_or(live)) return;  // every element of the workgroup is frozen (uniform exit)
+
+  const int  cx = t % Q, cy = t / Q;  // column
+  const int  ny = t % P, nz = t / P;  // x-line of nodes
+  const bool xline = t < P * P;
+  // E and E^T run on the node columns (cx, cy < P), like the column layout, so
+  // that a wave's loads and stores of u and v are coalesced along x
+  const bool node = live && cx < P && cy < P;
+  int        off[P];  // 2 * node + first_touch, nodes (cx, cy, z)
+  if (node) {
+#pragma unroll
+    for (int z = 0; z < P; z++) off[z] = offsets[e * P3 + (z * P + cy) * P + cx];
+  }
+  // the column's geometric factors, all loads issued at once and shared by the components
+  const T *qd = q_data + (live ? e : 0) * (int64_t)QC * Q3 + (cy * Q + cx);
+  T        sq[Q][QC];
+#pragma unroll
+  for (int qz = 0; qz < Q; qz++)
+#pragma unroll
+    for (int k = 0; k < QC; k++) sq[qz][k] = __builtin_nontemporal_load(&qd[k * Q3 + qz * Q * Q]);
+
+  for (int comp = 0; comp < NCOMP; comp++) {
+    const int64_t coff = (int64_t)comp * comp_stride;
+    if (comp > 0) __syncthreads();  // the previous component's last phase still reads S0
+    // ---- E: the node column gathers its P nodes into S0
+    if (node) {
+#pragma unroll
+      for (int z = 0; z < P; z++) S0[at(z, cy, cx)] = u[(off[z] >> 1) + coff];
+    }
+    __syncthreads();
+    // ---- B_x: x-line (ny, nz) of nodes, Q interpolated values into S1
+    if (xline) {
+      const auto gB = phase_ptr(mats);
+      T        in[P];
+#pragma unroll
+      for (int x = 0; x < P; x++) in[x] = S0[at(nz, ny, x)];
+#pragma unroll
+      for (int qx = 0; qx < Q; qx++) {
+        const auto m = kRowWise ? phase_ptr(mats + qx * P) : gB + qx * P;
+        T s = 0;
+#pragma unroll
+        for (int b = 0; b < P; b++) s += m[b] * in[b];
+        S1[at(nz, ny, qx)] = s;
+      }
+    }
+    __syncthreads();
+    // ---- B_y: y-line (qx, z), z < P
+    if (t < Q * P) {
+      const auto gB = phase_ptr(mats);
+      const int qx = t % Q, z = t / Q;
+      T         in[P];
+#pragma unroll
+      for (int b = 0; b < P; b++) in[b] = S1[at(z, b, qx)];
+#pragma unroll
+      for (int qy = 0; qy < Q; qy++) {
+        const auto m = kRowWise ? phase_ptr(mats + qy * P) : gB + qy * P;
+        T s = 0;
+#pragma unroll
+        for (int b = 0; b < P; b++) s += m[b] * in[b];
+        S0[at(z, qy, qx)] = s;
+      }
+    }
+    __syncthreads();
+    // ---- B_z on the column (cx, cy): uq[qz]
+    T uq[Q];
+    {
+      const auto gB = phase_ptr(mats);
+      T        col[P];
+#pragma unroll
+      for (int b = 0; b < P; b++) col[b] = S0[at(b, cy, cx)];
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) {
+        const auto m = kRowWise ? phase_ptr(mats + qz * P) : gB + qz * P;
+        T s = 0;
+#pragma unroll
+        for (int b = 0; b < P; b++) s += m[b] * col[b];
+        uq[qz] = s;
+      }
+    }
+    if constexpr (QF == kQfMass) {
+      // ---- D, then B_z^T back into the column's own S0 entries (read only by this thread)
+      const auto gB = phase_ptr(mats);
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) uq[qz] *= live ? sq[qz][0] : T(0);
+#pragma unroll
+      for (int b = 0; b < P; b++) {
+        const auto m = kRowWise ? phase_ptr(mats + b) : gB + b;
+        T s = 0;
+#pragma unroll
+        for (int qz = 0; qz < Q; qz++) s += m[qz * P] * uq[qz];
+        S0[at(b, cy, cx)] = s;
+      }
+    } else {
+      // ---- collocated derivatives: z on the column, x and y by lines through S1
+      const auto gD = phase_ptr(mats + Q * P);
+      T        dz[Q];
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) {
+        const auto m = kRowWise ? phase_ptr(mats + Q * P + qz * Q) : gD + qz * Q;
+        T s = 0;
+#pragma unroll
+        for (int c = 0; c < Q; c++) s += m[c] * uq[c];
+        dz[qz] = s;
+        S1[at(qz, cy, cx)] = uq[qz];
+      }
+      __syncthreads();
+      {
+        const auto gD = phase_ptr(mats + Q * P);
+        const int a = t % Q, qz = t / Q;  // x-line (qy = a, qz) and y-line (qx = a, qz)
+        T         in[Q];
+#pragma unroll
+        for (int c = 0; c < Q; c++) in[c] = S1[at(qz, a, c)];
+#pragma unroll
+        for (int qx = 0; qx < Q; qx++) {
+          const auto m = kRowWise ? phase_ptr(mats + Q * P + qx * Q) : gD + qx * Q;
+          T s = 0;
+#pragma unroll
+          for (int c = 0; c < Q; c++) s += m[c] * in[c];
+          S2[at(qz, a, qx)] = s;
+        }
+#pragma unroll
+        for (int c = 0; c < Q; c++) in[c] = S1[at(qz, c, a)];
+#pragma unroll
+        for (int qy = 0; qy < Q; qy++) {
+          const auto m = kRowWise ? phase_ptr(mats + Q * P + qy * Q) : gD + qy * Q;
+          T s = 0;
+#pragma unroll
+          for (int c = 0; c < Q; c++) s += m[c] * in[c];
+          S0[at(qz, qy, a)] = s;
+        }
+      }
+      __syncthreads();
+      // ---- pointwise (Poisson3DApply, times kappa for the kappa-weighted operator) on the column
+      T vz[Q];
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) {
+        T vx, vy, vzz;
+        poisson_regs<T>(sq[qz], S2[at(qz, cy, cx)], S0[at(qz, cy, cx)], dz[qz], vx, vy, vzz);
+        if constexpr (QF == kQfDiffusionKappa) {
+          vx *= sq[qz][6];
+          vy *= sq[qz][6];
+          vzz *= sq[qz][6];
+        }
+        if (!live) vx = vy = vzz = 0;
+        S2[at(qz, cy, cx)] = vx;
+        S0[at(qz, cy, cx)] = vy;
+        vz[qz]             = vzz;
+      }
+      __syncthreads();
+      // ---- D_x^T by x-lines into S1, D_y^T by y-lines into S3
+      {
+        const auto gD = phase_ptr(mats + Q * P);
+        const int a = t % Q, qz = t / Q;
+        T         in[Q];
+#pragma unroll
+        for (int c = 0; c < Q; c++) in[c] = S2[at(qz, a, c)];
+#pragma unroll
+        for (int qx = 0; qx < Q; qx++) {
+          const auto m = kRowWise ? phase_ptr(mats + Q * P + qx) : gD + qx;
+          T s = 0;
+#pragma unroll
+          for (int c = 0; c < Q; c++) s += m[c * Q] * in[c];
+          S1[at(qz, a, qx)] = s;
+        }
+#pragma unroll
+        for (int c = 0; c < Q; c++) in[c] = S0[at(qz, c, a)];
+#pragma unroll
+        for (int qy = 0; qy < Q; qy++) {
+          const auto m = kRowWise ? phase_ptr(mats + Q * P + qy) : gD + qy;
+          T s = 0;
+#pragma unroll
+          for (int c = 0; c < Q; c++) s += m[c * Q] * in[c];
+          S3[at(qz, qy, a)] = s;
+        }
+      }
+      __syncthreads();
+      // ---- sum of the three transposes on the column, then B_z^T into S0
+      const auto gDz = phase_ptr(mats + Q * P);
+      T        w[Q];
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) {
+        const auto m = kRowWise ? phase_ptr(mats + Q * P + qz) : gDz + qz;
+        T s = S1[at(qz, cy, cx)] + S3[at(qz, cy, cx)];
+#pragma unroll
+        for (int c = 0; c < Q; c++) s += m[c * Q] * vz[c];
+        w[qz] = s;
+      }
+      const auto gBz = phase_ptr(mats);
+#pragma unroll
+      for (int b = 0; b < P; b++) {
+        const auto m = kRowWise ? phase_ptr(mats + b) : gBz + b;
+        T s = 0;
+#pragma unroll
+        for (int qz = 0; qz < Q; qz++) s += m[qz * P] * w[qz];
+        S0[at(b, cy, cx)] = s;
+      }
+    }
+    __syncthreads();
+    // ---- B_y^T: y-line (qx, z), z < P, into S1
+    if (t < Q * P) {
+      const auto gB = phase_ptr(mats);
+      const int qx = t % Q, z = t / Q;
+      T         in[Q];
+#pragma unroll
+      for (int a = 0; a < Q; a++) in[a] = S0[at(z, a, qx)];
+#pragma unroll
+      for (int y = 0; y < P; y++) {
+        const auto m = kRowWise ? phase_ptr(mats + y) : gB + y;
+        T s = 0;
+#pragma unroll
+        for (int a = 0; a < Q; a++) s += m[a * P] * in[a];
+        S1[at(z, y, qx)] = s;
+      }
+    }
+    __syncthreads();
+    // ---- B_x^T: x-line (ny, nz) of nodes, into S0 (last read by B_y^T)
+    if (xline) {
+      const auto gB = phase_ptr(mats);
+      T        in[Q];
+#pragma unroll
+      for (int a = 0; a < Q; a++) in[a] = S1[at(nz, ny, a)];
+#pragma unroll
+      for (int x = 0; x < P; x++) {
+        const auto m = kRowWise ? phase_ptr(mats + x) : gB + x;
+        T acc = 0;
+#pragma unroll
+        for (int a = 0; a < Q; a++) acc += m[a * P] * in[a];
+        S0[at(nz, ny, x)] = acc;
+      }
+    }
+    __syncthreads();
+    // ---- E^T from the node columns
+    if (node) {
+      T s[P];
+#pragma unroll
+      for (int z = 0; z < P; z++) s[z] = S0[at(z, cy, cx)];
+      if (scatter == kScatterEvec) {  // tpos gives each contribution's slot in the E-vector sorted by L entry
+        int32_t pos[P];
+#pragma unroll
+        for (int z = 0; z < P; z++) pos[z] = tpos[(e * NCOMP + comp) * (int64_t)P3 + (z * P + cy) * P + cx];
+#pragma unroll
+        for (int z = 0; z < P; z++) v[pos[z]] = s[z];
+      } else if (scatter == kScatterAtomic) {
+#pragma unroll
+        for (int z = 0; z < P; z++) unsafeAtomicAdd(&v[(off[z] >> 1) + coff], s[z]);
+      } else {
+        const bool overwrite = scatter == kScatterOverwrite;
+        T          old[P];
+#pragma unroll
+        for (int z = 0; z < P; z++) old[z] = (overwrite && (off[z] & 1)) ? T(0) : v[(off[z] >> 1) + coff];
+#pragma unroll
+        for (int z = 0; z < P; z++) v[(off[z] >> 1) + coff] = (overwrite && (off[z] & 1)) ? s[z] : old[z] + s[z];
+      }
+    }
+  }
+}
+
+}  // namespace mi355

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "hex_fused.hpp"
+#include "hex_lines.hpp"
 #include "mi355_common.h"
 
 struct CeedMi355Operator_private;
@@ -20,9 +21,13 @@ int transpose_gather(const CeedMi355Operator_private *op, void *v, int overwrite
 // Kernel launcher for one (P, Q, num_comp, QF, scalar type); nullptr when no instantiation exists.
 ApplyFn pick_apply_f64(int P, int Q, int num_comp, int qf);
 ApplyFn pick_apply_f32(int P, int Q, int num_comp, int qf);
+// Thread layouts of the fused kernel: z-columns with x/y contractions read
+// point by point from LDS (hex_apply_kernel), or whole 1D lines per thread
+// for the x/y contractions (hex_apply_lines_kernel).
+enum { kLayoutColumns = 0, kLayoutLines = 1 };
 // Elements per workgroup giving ~256 threads and at most 64 KiB of LDS.
-int default_epb(int P, int Q, int qf, int scalar_bytes);
-int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes);
+int default_epb(int P, int Q, int qf, int scalar_bytes, int layout);
+int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes, int layout);
 
 }  // namespace mi355
 
@@ -31,6 +36,7 @@ struct CeedMi355Operator_private {
   int32_t num_elem, comp_stride;
   int64_t l_size;
   int     epb;
+  int     layout     = 0;      // mi355::kLayoutColumns or kLayoutLines
   bool    covers_all = false;  // every L entry is reached by some element: Apply needs no zero fill
   int     scatter    = 0;      // CEED_MI355_PLAN_* E^T mode: 0 transpose map, COLOR, ATOMIC
   // transpose offset map (scatter == 0): contribution (elem, comp, node) is
@@ -56,19 +62,25 @@ namespace mi355 {
 
 template <int P, int Q, int NC, int QF, typename T>
 int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v, int overwrite, hipStream_t s) {
-  const int    nt  = Q * Q;
-  const int    epb = op->epb;
-  const size_t lds = sizeof(T) * (size_t)hex_lds_entries<P, Q, QF>(epb);
-  const bool tmap = op->scatter == 0;
+  const int    nt    = Q * Q;
+  const int    epb   = op->epb;
+  const bool   lines = op->layout == kLayoutLines;
+  const size_t lds   = sizeof(T) * (size_t)(lines ? hex_lines_lds_entries<P, Q, QF>(epb) : hex_lds_entries<P, Q, QF>(epb));
+  const bool   tmap  = op->scatter == 0;
   for (size_t c = 0; c + 1 < op->color_start.size(); c++) {
     const int64_t b = op->color_start[c], e = op->color_start[c + 1];
     if (e <= b) continue;
     const int64_t grid    = (e - b + epb - 1) / epb;
     const int     scatter = tmap ? kScatterEvec
                                  : (op->scatter == CEED_MI355_PLAN_ATOMIC ? kScatterAtomic : (overwrite ? kScatterOverwrite : kScatterAdd));
-    hex_apply_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, scatter, op->d_offsets, op->comp_stride, (const T *)op->d_qdata,
-                                                                          (const T *)op->d_mats, (const T *)u, tmap ? (T *)op->d_evec : (T *)v,
-                                                                          op->d_tpos, op->d_elem_sys, op->d_sys_active);
+    if (lines)
+      hex_apply_lines_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(
+          b, e, scatter, op->d_offsets, op->comp_stride, (const T *)op->d_qdata, (const T *)op->d_mats, (const T *)u,
+          tmap ? (T *)op->d_evec : (T *)v, op->d_tpos, op->d_elem_sys, op->d_sys_active);
+    else
+      hex_apply_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, scatter, op->d_offsets, op->comp_stride, (const T *)op->d_qdata,
+                                                                            (const T *)op->d_mats, (const T *)u, tmap ? (T *)op->d_evec : (T *)v,
+                                                                            op->d_tpos, op->d_elem_sys, op->d_sys_active);
   }
   MI355_LAUNCH_CHECK();
   if (tmap) return transpose_gather(op, v, overwrite, s);

@@ -14,15 +14,19 @@
 namespace mi355 {
 void collocated_grad(int P, int Q, const double *interp, const double *grad, double *collo);
 
-int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes) {
+int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes, int layout) {
+  if (layout == kLayoutLines) return scalar_bytes * epb * (qf == kQfMass ? 2 : 4) * Q * Q * Q;
   const int slabs = qf == kQfMass ? 2 : 3;
   return scalar_bytes * (Q * P + Q * Q + epb * slabs * Q * Q * Q);
 }
 
-int default_epb(int P, int Q, int qf, int scalar_bytes) {
-  int epb = 256 / (Q * Q);
+int default_epb(int P, int Q, int qf, int scalar_bytes, int layout) {
+  // as many elements as fit 256 threads; diffusion in the line layout about
+  // two waves instead (its thirteen barriers wait on fewer waves; near the
+  // autotuner's choices on MI355X)
+  int epb = (layout == kLayoutLines && qf != kQfMass) ? 128 / (Q * Q) : 256 / (Q * Q);
   if (epb < 1) epb = 1;
-  while (epb > 1 && lds_bytes(P, Q, qf, epb, scalar_bytes) > 64 * 1024) epb--;
+  while (epb > 1 && lds_bytes(P, Q, qf, epb, scalar_bytes, layout) > 64 * 1024) epb--;
   return epb;
 }
 }  // namespace mi355
@@ -207,22 +211,31 @@ int color_elements(const int32_t *offsets, int64_t num_elem, int es, int nc, int
   return CEED_MI355_SUCCESS;
 }
 
-// Autotuner results: elements per workgroup by (P, Q, num_comp, pointwise operator, scalar type, E^T mode).
-std::mutex                                                     g_tune_mutex;
-std::map<std::tuple<int, int, int, int, int, int>, int>        g_tuned;
+// Autotuner results: (thread layout, elements per workgroup) by (P, Q,
+// num_comp, pointwise operator, scalar type, E^T mode).
+std::mutex                                                          g_tune_mutex;
+std::map<std::tuple<int, int, int, int, int, int>, std::pair<int, int>> g_tuned;
 
 std::tuple<int, int, int, int, int, int> tune_key(const CeedMi355Operator op) {
   return std::make_tuple(op->P, op->Q, op->num_comp, op->qf, op->scalar_type, op->scatter);
 }
 
-int tuned_epb(const CeedMi355Operator op) {
+bool tuned_launch(const CeedMi355Operator op, int *layout, int *epb) {
   std::lock_guard<std::mutex> lock(g_tune_mutex);
   auto                        it = g_tuned.find(tune_key(op));
-  return it == g_tuned.end() ? 0 : it->second;
+  if (it == g_tuned.end()) return false;
+  *layout = it->second.first;
+  *epb    = it->second.second;
+  return true;
+}
+
+bool admissible(int P, int Q, int qf, int scalar_bytes, int layout, int epb) {
+  return (layout == mi355::kLayoutColumns || layout == mi355::kLayoutLines) && epb >= 1 && epb * Q * Q <= 256 &&
+         mi355::lds_bytes(P, Q, qf, epb, scalar_bytes, layout) <= 64 * 1024;
 }
 
 bool admissible_epb(const CeedMi355Operator op, int epb) {
-  return epb >= 1 && epb * op->Q * op->Q <= 256 && mi355::lds_bytes(op->P, op->Q, op->qf, epb, op->scalar_bytes) <= 64 * 1024;
+  return admissible(op->P, op->Q, op->qf, op->scalar_bytes, op->layout, epb);
 }
 
 int64_t elem_block(const CeedMi355Operator op) {
@@ -306,14 +319,23 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
   else if (flags & CEED_MI355_PLAN_COLOR) op->scatter = CEED_MI355_PLAN_COLOR;
   else if (flags & CEED_MI355_PLAN_TMAP) op->scatter = 0;
   else op->scatter = P_1d >= 6 ? 0 : CEED_MI355_PLAN_COLOR;
-  op->epb     = mi355::default_epb(P_1d, Q_1d, qf, op->scalar_bytes);
+  // launch: the tuned (layout, elements per workgroup) of this kind of plan if the
+  // autotuner has run on one in this process, else the line layout (faster
+  // than the column layout at every order measured on MI355X, equal at p <= 2)
+  // with its default workgroup size; CEED_MI355_LAYOUT / CEED_MI355_EPB override
+  op->layout = mi355::kLayoutLines;
+  op->epb    = mi355::default_epb(P_1d, Q_1d, qf, op->scalar_bytes, op->layout);
   {
-    const int tuned = tuned_epb(op);
-    if (tuned > 0) op->epb = tuned;
+    int layout, epb;
+    if (tuned_launch(op, &layout, &epb)) op->layout = layout, op->epb = epb;
+  }
+  if (const char *env = getenv("CEED_MI355_LAYOUT")) {
+    const int l = atoi(env);
+    if (l == mi355::kLayoutColumns || l == mi355::kLayoutLines) op->layout = l, op->epb = mi355::default_epb(P_1d, Q_1d, qf, op->scalar_bytes, l);
   }
   if (const char *env = getenv("CEED_MI355_EPB")) {
-    int v = atoi(env);
-    if (v >= 1 && mi355::lds_bytes(P_1d, Q_1d, qf, v, op->scalar_bytes) <= 64 * 1024 && v * Q_1d * Q_1d <= 256) op->epb = v;
+    const int v = atoi(env);
+    if (admissible(P_1d, Q_1d, qf, op->scalar_bytes, op->layout, v)) op->epb = v;
   }
 
   const int            es = P_1d * P_1d * P_1d;
@@ -527,6 +549,24 @@ extern "C" int CeedMi355OperatorSetLaunch(CeedMi355Operator op, int32_t elems_pe
   return CEED_MI355_SUCCESS;
 }
 
+extern "C" int CeedMi355OperatorSetLayout(CeedMi355Operator op, int32_t layout, int32_t elems_per_block) {
+  MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
+  MI355_CHECK(layout == mi355::kLayoutColumns || layout == mi355::kLayoutLines, CEED_MI355_ERROR_BACKEND, "unknown thread layout %d", layout);
+  const int epb = elems_per_block > 0 ? elems_per_block : mi355::default_epb(op->P, op->Q, op->qf, op->scalar_bytes, layout);
+  MI355_CHECK(admissible(op->P, op->Q, op->qf, op->scalar_bytes, layout, epb), CEED_MI355_ERROR_BACKEND,
+              "%d elements per workgroup do not fit layout %d (at most 256 threads and 64 KiB of LDS for P=%d Q=%d)", epb, layout, op->P, op->Q);
+  op->layout = layout;
+  op->epb    = epb;
+  return CEED_MI355_SUCCESS;
+}
+
+extern "C" int CeedMi355OperatorGetLaunch(CeedMi355Operator op, int32_t *layout, int32_t *elems_per_block) {
+  MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
+  if (layout) *layout = op->layout;
+  if (elems_per_block) *elems_per_block = op->epb;
+  return CEED_MI355_SUCCESS;
+}
+
 namespace {
 template <typename T>
 __global__ void __launch_bounds__(256) tune_fill(int64_t n, T *__restrict__ x) {
@@ -545,25 +585,29 @@ extern "C" int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int
   MI355_HIP(hipEventCreate(&t1));
   if (op->scalar_type == CEED_MI355_FP64) tune_fill<double><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, (double *)u);
   else tune_fill<float><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, (float *)u);
-  const int start = op->epb;
-  int       best  = start;
-  float     bms   = 1e30f;
-  int       err   = CEED_MI355_SUCCESS;
-  for (int epb = 1; epb * op->Q * op->Q <= 256 && !err; epb++) {
-    if (!admissible_epb(op, epb)) continue;
-    op->epb = epb;
-    err     = CeedMi355OperatorApply(op, u, v, s);  // warm-up
-    if (err) break;
-    hipEventRecord(t0, s);
-    for (int r = 0; r < reps && !err; r++) err = CeedMi355OperatorApply(op, u, v, s);
-    hipEventRecord(t1, s);
-    hipEventSynchronize(t1);
-    float ms = 0;
-    hipEventElapsedTime(&ms, t0, t1);
-    ms /= reps;
-    if (ms < bms) bms = ms, best = epb;
+  const int start = op->epb, start_layout = op->layout;
+  int       best = start, best_layout = start_layout;
+  float     bms  = 1e30f;
+  int       err  = CEED_MI355_SUCCESS;
+  for (int layout = mi355::kLayoutColumns; layout <= mi355::kLayoutLines && !err; layout++) {
+    for (int epb = 1; epb * op->Q * op->Q <= 256 && !err; epb++) {
+      if (!admissible(op->P, op->Q, op->qf, op->scalar_bytes, layout, epb)) continue;
+      op->layout = layout;
+      op->epb    = epb;
+      err        = CeedMi355OperatorApply(op, u, v, s);  // warm-up
+      if (err) break;
+      hipEventRecord(t0, s);
+      for (int r = 0; r < reps && !err; r++) err = CeedMi355OperatorApply(op, u, v, s);
+      hipEventRecord(t1, s);
+      hipEventSynchronize(t1);
+      float ms = 0;
+      hipEventElapsedTime(&ms, t0, t1);
+      ms /= reps;
+      if (ms < bms) bms = ms, best = epb, best_layout = layout;
+    }
   }
-  op->epb = err ? start : best;
+  op->epb    = err ? start : best;
+  op->layout = err ? start_layout : best_layout;
   hipEventDestroy(t0);
   hipEventDestroy(t1);
   hipFree(u);
@@ -571,7 +615,7 @@ extern "C" int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int
   if (err) return err;
   {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
-    g_tuned[tune_key(op)] = best;
+    g_tuned[tune_key(op)] = std::make_pair(best_layout, best);
   }
   if (best_epb) *best_epb = best;
   if (best_ms) *best_ms = bms;

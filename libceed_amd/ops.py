@@ -14,11 +14,12 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import call
+from ._lib import CeedMi355Error, call  # noqa: F401  (CeedMi355Error re-exported)
 
 NOTRANSPOSE, TRANSPOSE = 0, 1
 EVAL_NONE, EVAL_INTERP, EVAL_GRAD, EVAL_WEIGHT = 0, 1, 2, 16
 QF_MASS, QF_DIFFUSION, QF_DIFFUSION_KAPPA = 0, 1, 2
+LAYOUT_COLUMNS, LAYOUT_LINES = 0, 1  # thread layouts of the fused kernel
 FP64, FP32 = 0, 1
 # E^T of the fused plan: "auto" (transpose map for P >= 6, colours below), transpose offset map,
 # element colours, hardware atomics
@@ -220,8 +221,18 @@ class HexOperator:
     def set_launch(self, elems_per_block: int) -> None:
         call("CeedMi355OperatorSetLaunch", self._h, int(elems_per_block))
 
+    def set_layout(self, layout: int, elems_per_block: int = 0) -> None:
+        """Thread layout of the fused kernel (LAYOUT_COLUMNS or LAYOUT_LINES); elems_per_block 0: as many as fit."""
+        call("CeedMi355OperatorSetLayout", self._h, int(layout), int(elems_per_block))
+
+    def launch(self) -> tuple[int, int]:
+        """(thread layout, elements per workgroup) of the fused kernel."""
+        layout, epb = C.c_int32(), C.c_int32()
+        call("CeedMi355OperatorGetLaunch", self._h, C.byref(layout), C.byref(epb))
+        return layout.value, epb.value
+
     def autotune(self, reps: int = 5) -> tuple[int, float]:
-        """Pick the fastest elements-per-workgroup for this plan; returns (epb, ms per apply)."""
+        """Pick the fastest (thread layout, elements per workgroup) for this plan; returns (epb, ms per apply)."""
         best, ms = C.c_int32(), C.c_double()
         call("CeedMi355OperatorAutotune", self._h, int(reps), C.byref(best), C.byref(ms), _stream())
         return best.value, ms.value

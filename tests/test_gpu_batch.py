@@ -39,8 +39,8 @@ def test_batch_converges_and_matches_single_solves(batch):
     st = batch.solve(b, x, max_iter=3000, rtol=1e-10, check_every=10)
     assert np.all(st["rel_residual"] <= 1e-10), st["rel_residual"]
     assert np.all(st["converged_at_check"] >= 0)
-    for (kind, P, Q), (epb, ms) in batch.tuning.items():
-        assert 1 <= epb and epb * Q * Q <= 256 and ms > 0
+    for (kind, P, Q), (layout, epb, ms) in batch.tuning.items():
+        assert layout in (ops.LAYOUT_COLUMNS, ops.LAYOUT_LINES) and 1 <= epb and epb * Q * Q <= 256 and ms > 0
     for i, s in enumerate(batch.systems):
         r = batch.range(i)
         op = ops.HexOperator.from_mesh(s.kind, s.mesh, Q=s.mesh.P + 1)

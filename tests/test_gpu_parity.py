@@ -183,6 +183,45 @@ def test_operator_vs_oracle(cuda, kind, nc, n, p, Q):
         assert rel_l2(host(w), w0 + v_ref) < RTOL64, det
 
 
+@pytest.mark.parametrize("kind,nc,n,p,Q,fp32", [(1, 1, 4, 4, 6, False), (1, 3, 3, 3, 5, False), (0, 1, 4, 6, 8, False), (1, 1, 2, 8, 10, False),
+                                                 (0, 3, 3, 2, 3, False), (1, 1, 5, 1, 2, False), (1, 1, 3, 5, 6, True), (0, 1, 3, 4, 5, True),
+                                                 (2, 1, 3, 4, 5, False), (2, 1, 2, 6, 8, False)])
+def test_thread_layouts_identical(cuda, kind, nc, n, p, Q, fp32):
+    """Both thread layouts of the fused kernel (z-columns, 1D lines), every admissible elements-per-workgroup count:
+    bitwise identical results (so the autotuner's choice never changes them) and oracle parity."""
+    from libceed_amd.mesh import quadrature_points, smooth_random_field
+
+    m = hex_mesh(n, p, seed=3 * n + p)
+    rng = np.random.default_rng(p + nc)
+    u = rng.uniform(-1, 1, nc * m.num_nodes)
+    qd_ref = oracle.hex_build_qdata(min(kind, 1), m, Q)
+    if kind == ops.QF_DIFFUSION_KAPPA:
+        kappa = np.exp(smooth_random_field(quadrature_points(m, oracle.gauss(Q)[0]), seed=p))
+        v_ref = oracle.hex_apply_kappa(m.num_elem, p + 1, Q, m.offsets, qd_ref, kappa, u)
+    else:
+        v_ref = oracle.hex_apply(kind, nc, m.num_elem, p + 1, Q, m.offsets, m.num_nodes, qd_ref, u)
+    dtype = torch.float32 if fp32 else torch.float64
+    for det in ("tmap", "color"):
+        op = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, dtype=dtype, scatter=det)
+        if kind == ops.QF_DIFFUSION_KAPPA:
+            op.set_kappa(dev(kappa))
+        outs = []
+        for layout in (ops.LAYOUT_COLUMNS, ops.LAYOUT_LINES):
+            for epb in range(1, 256 // (Q * Q) + 1):
+                try:
+                    op.set_layout(layout, epb)
+                except ops.CeedMi355Error:
+                    continue  # does not fit the LDS budget
+                print("layout", layout, "epb", epb, det, flush=True)
+                v = torch.full((nc * m.num_nodes,), 3.0, dtype=dtype, device="cuda")
+                op.apply(dev(u, dtype), v)
+                outs.append((layout, epb, host(v)))
+        assert {o[0] for o in outs} == {ops.LAYOUT_COLUMNS, ops.LAYOUT_LINES}
+        for layout, epb, o in outs[1:]:
+            assert np.array_equal(o, outs[0][2]), (det, layout, epb)
+        assert rel_l2(outs[0][2], v_ref) < (RTOL32 if fp32 else RTOL64), det
+
+
 def test_mass_config_repeated_apply(cuda):
     """Mass operator, 16^3 elements, p = 3, Q = 5: v = M u applied 100 times, bitwise repeatable, oracle parity."""
     m = hex_mesh(16, 3, seed=1)
