@@ -233,6 +233,19 @@ def run_adjoint(dofs: float = 4e6, p: int = 4, seed: int = 0, directions: int = 
             "fd_max_rel_error": {f"h={k:g}": v for k, v in worst.items()}}
 
 
+def run_drop_in(p: int = 4, n: int = 50, reps: int = 50, num_comp: int = 1):
+    """CeedOperatorApply through the libCEED C API on /gpu/hip/mi355 (tests/c/bp-apply.c: gallery Poisson3DBuild +
+    Poisson3DApply, library defaults, no tuning), in a child process."""
+    exe = ROOT / "tests" / "c" / "bin" / "bp-apply"
+    if not exe.exists():
+        return None
+    res = subprocess.run([str(exe), "/gpu/hip/mi355", str(p), str(n), str(reps), str(num_comp)], capture_output=True, text=True,
+                         timeout=600)
+    if res.returncode != 0:
+        return {"error": res.stderr.strip()[-300:]}
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
 def run_batch(count: int = 256, rtol: float = 1e-8, seed: int = 0) -> dict:
     """Batched uneven solves (one fused plan per (operator, P, Q) group, hipGraph-captured iterations) against
     one CG per mesh, solved one after the other."""
@@ -411,6 +424,11 @@ def main() -> None:
             sizes[label] = {"dofs": r["dofs"], "num_comp": nc, "fp64_gdofs": round(r["gdofs"], 3), "fp64_hbm_frac": round(r["hbm_frac"], 4),
                             "launch": [r["layout"], r["elems_per_block"]]}
             print(f"# {label}: {sizes[label]}", file=sys.stderr, flush=True)
+    drop_in = {}
+    if not args.no_sweep:
+        for nc in (1, 3):
+            drop_in[f"p4_n50_ncomp{nc}"] = run_drop_in(num_comp=nc)
+            print(f"# drop-in C API: {drop_in[f'p4_n50_ncomp{nc}']}", file=sys.stderr, flush=True)
     adjoint = None
     if not args.no_sweep:
         adjoint = run_adjoint()
@@ -455,6 +473,7 @@ def main() -> None:
         "cpu_baseline": cpu,
         "sweep": sweep,
         "sizes": sizes,
+        "drop_in_c_api": drop_in,
         "adjoint": adjoint,
         "cg": cg,
         "batched_solves": batch,
