@@ -181,6 +181,18 @@ def run_cg_mixed(p: int, dofs: float, ks=(10, 25, 50), rtol: float = 1e-10) -> d
     best = max(mixed, key=lambda k: mixed[k]["speedup"])
     out["best_k"] = int(best)
     out["speedup"] = mixed[best]["speedup"]
+    # Jacobi preconditioning with the assembled diagonal (FP64 and mixed at the best k)
+    diag = torch.zeros_like(b)
+    ms_diag, _ = time_gpu(lambda: op.assemble_diagonal(diag), 5, 1)
+    jac = {"ms_assemble_diagonal": round(ms_diag, 4)}
+    for name, solver, every in (("fp64", ops.CG(op), 50), ("fp32_refine", ops.CG(op32, op), int(best))):
+        solver.set_jacobi(diag)
+        x, st, r = timed(solver, every)
+        d = x - x64
+        r["rel_error_vs_fp64"] = (torch.linalg.norm(d - d.mean()) / torch.linalg.norm(xc)).item()
+        r["speedup_vs_plain_fp64_cg"] = round(out["fp64"]["seconds"] / r["seconds"], 3)
+        jac[name] = r
+    out["jacobi"] = jac
     return out
 
 

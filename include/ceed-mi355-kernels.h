@@ -251,6 +251,13 @@ int CeedMi355CGCreate(CeedMi355Operator op, CeedMi355Operator op64, CeedMi355CG 
  * returns when the solve is done. */
 int CeedMi355CGSolve(CeedMi355CG cg, const double *b, double *x, int32_t max_iter, double rtol, int32_t check_every, int use_graph, void *stream,
                      CeedMi355CGStats *stats_host);
+/* Jacobi preconditioning for later solves: z = D^-1 r with D the operator's
+ * diagonal (FP64, length l_size, e.g. from CeedMi355OperatorAssembleDiagonal;
+ * entries must be non-zero); NULL switches it off.  The reference's PETSc
+ * examples build this preconditioner from CeedOperatorLinearAssembleDiagonal
+ * (examples/petsc/src/matops.c:54, interface/ceed-preconditioning.c:2395).
+ * The stopping test stays on the unpreconditioned residual. */
+int CeedMi355CGSetJacobi(CeedMi355CG cg, const double *diag, void *stream);
 int CeedMi355CGDestroy(CeedMi355CG *cg);
 
 /* ------------------------------------------------------------------------ */

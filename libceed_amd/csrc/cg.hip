@@ -15,11 +15,17 @@
 // Sleijpen and van der Vorst, Computing 56 (1996) 141-163), so the search
 // directions carry on and the attainable accuracy is FP64's.
 //
+// Jacobi preconditioning (optional, CeedMi355CGSetJacobi): z = D^-1 r with D
+// the operator's assembled diagonal (CeedMi355OperatorAssembleDiagonal, the
+// input the reference's PETSc examples build their Jacobi preconditioner
+// from: examples/petsc/src/matops.c:54); then rho = r . z drives alpha and
+// beta while r . r drives the stopping test.
+//
 // Per iteration (n = L-vector length): the operator apply, then
-//   k_dot:       p . Ap                          (reads 2n)
-//   k_update_r:  r -= alpha Ap, partial r . r    (reads 2n, writes n)
-//   k_update_px: x += alpha p, p = r + beta p    (reads 3n, writes 2n)
-// plus two single-block reductions that finish the dot products and form
+//   k_dot:       p . Ap                                        (reads 2n)
+//   k_update_r:  r -= alpha Ap [z = D^-1 r], partial r.r [r.z]  (reads 2n [3n], writes n [2n])
+//   k_update_px: x += alpha p, p = z + beta p  (z = r unpreconditioned)  (reads 3n, writes 2n)
+// plus single-block reductions that finish the dot products and form
 // alpha and beta.
 #include <math.h>
 #include <string.h>
@@ -32,8 +38,8 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kBlocks  = 1024;  // fixed reduction grid: bitwise-deterministic sums
 
-// device scalar slots
-enum { kRR = 0, kPAP, kAlpha, kBeta, kRRTrue, kBB, kNumScal = 8 };
+// device scalar slots: kRho = r . z (= r . r unpreconditioned) drives alpha and beta, kRR = r . r the stopping test
+enum { kRR = 0, kPAP, kAlpha, kBeta, kRRTrue, kBB, kRho, kNumScal = 8 };
 
 __device__ double block_sum(double v) {
   __shared__ double s[kThreads];
@@ -54,7 +60,8 @@ __global__ void __launch_bounds__(kThreads) k_dot(int64_t n, const A *__restrict
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// Finish a dot product into scal[slot]; mode 1: alpha = RR / PAP; mode 2: beta = new / RR, RR = new.
+// Finish a dot product into scal[slot]; mode 1: alpha = Rho / PAP; mode 2: beta = new / Rho, Rho = new
+// (and scal[slot] = new, slot kRR when unpreconditioned, kRho otherwise).
 __global__ void __launch_bounds__(kThreads) k_finish(int nb, const double *__restrict__ partial, double *__restrict__ scal, int slot, int mode) {
   double acc = 0;
   for (int i = threadIdx.x; i < nb; i += kThreads) acc += partial[i];
@@ -62,10 +69,11 @@ __global__ void __launch_bounds__(kThreads) k_finish(int nb, const double *__res
   if (threadIdx.x == 0) {
     if (mode == 1) {
       scal[kPAP]   = s;
-      scal[kAlpha] = s != 0.0 ? scal[kRR] / s : 0.0;
+      scal[kAlpha] = s != 0.0 ? scal[kRho] / s : 0.0;
     } else if (mode == 2) {
-      scal[kBeta] = scal[kRR] != 0.0 ? s / scal[kRR] : 0.0;
-      scal[kRR]   = s;
+      scal[kBeta] = scal[kRho] != 0.0 ? s / scal[kRho] : 0.0;
+      scal[kRho]  = s;
+      scal[slot]  = s;
     } else {
       scal[slot] = s;
     }
@@ -87,7 +95,41 @@ __global__ void __launch_bounds__(kThreads) k_update_r(int64_t n, const double *
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// x += alpha p;  p = r + beta p
+// r -= alpha Ap, z = invd r, and the partial sums of r . r and r . z
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_update_r_jacobi(int64_t n, const double *__restrict__ scal, T *__restrict__ r, const T *__restrict__ Ap,
+                                                              const T *__restrict__ invd, T *__restrict__ z, double *__restrict__ partial_rr,
+                                                              double *__restrict__ partial_rz) {
+  const T alpha = (T)scal[kAlpha];
+  double  rr = 0, rz = 0;
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    const T ri = r[i] - alpha * Ap[i];
+    const T zi = invd[i] * ri;
+    r[i]       = ri;
+    z[i]       = zi;
+    rr += (double)ri * (double)ri;
+    rz += (double)ri * (double)zi;
+  }
+  const double s = block_sum(rr);
+  if (threadIdx.x == 0) partial_rr[blockIdx.x] = s;
+  __syncthreads();  // block_sum's shared buffer is reused
+  const double t = block_sum(rz);
+  if (threadIdx.x == 0) partial_rz[blockIdx.x] = t;
+}
+
+// z = invd r
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_scale(int64_t n, const T *__restrict__ invd, const T *__restrict__ r, T *__restrict__ z) {
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) z[i] = invd[i] * r[i];
+}
+
+// invd = 1 / diag (in the inner scalar type)
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_invert(int64_t n, const double *__restrict__ diag, T *__restrict__ invd) {
+  for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) invd[i] = (T)(1.0 / diag[i]);
+}
+
+// x += alpha p;  p = z + beta p  (z = r unpreconditioned)
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_update_px(int64_t n, const double *__restrict__ scal, T *__restrict__ x, T *__restrict__ p,
                                                         const T *__restrict__ r) {
@@ -143,8 +185,10 @@ struct CeedMi355CG_private {
   int64_t           n     = 0;
   hipStream_t       stream = nullptr;
   hipEvent_t        ev     = nullptr;
-  double           *scal = nullptr, *h_scal = nullptr, *partial = nullptr;
+  double           *scal = nullptr, *h_scal = nullptr, *partial = nullptr, *partial2 = nullptr;
   void             *r = nullptr, *p = nullptr, *Ap = nullptr, *dx = nullptr;  // inner scalar type
+  bool              jacobi = false;
+  void             *invd = nullptr, *z = nullptr;  // Jacobi: 1 / diag and z = invd r, inner scalar type
   double           *r64 = nullptr, *Ap64 = nullptr;                            // mixed only
   hipGraphExec_t    graph = nullptr;
   int               graph_len = 0;
@@ -161,14 +205,20 @@ template <typename T>
 int iteration(CeedMi355CG cg, T *x) {
   hipStream_t s  = cg->stream;
   const int   nb = nblocks(cg->n);
-  T *r = (T *)cg->r, *p = (T *)cg->p, *Ap = (T *)cg->Ap;
+  T *r = (T *)cg->r, *p = (T *)cg->p, *Ap = (T *)cg->Ap, *z = cg->jacobi ? (T *)cg->z : r;
   int ierr = CeedMi355OperatorApply(cg->op, p, Ap, s);
   if (ierr) return ierr;
   k_dot<T, T><<<nb, kThreads, 0, s>>>(cg->n, p, Ap, cg->partial);
   k_finish<<<1, kThreads, 0, s>>>(nb, cg->partial, cg->scal, kPAP, 1);
-  k_update_r<T><<<nb, kThreads, 0, s>>>(cg->n, cg->scal, r, Ap, cg->partial);
-  k_finish<<<1, kThreads, 0, s>>>(nb, cg->partial, cg->scal, kRR, 2);
-  k_update_px<T><<<nb, kThreads, 0, s>>>(cg->n, cg->scal, x, p, r);
+  if (cg->jacobi) {
+    k_update_r_jacobi<T><<<nb, kThreads, 0, s>>>(cg->n, cg->scal, r, Ap, (const T *)cg->invd, z, cg->partial, cg->partial2);
+    k_finish<<<1, kThreads, 0, s>>>(nb, cg->partial2, cg->scal, kRho, 2);
+    k_finish<<<1, kThreads, 0, s>>>(nb, cg->partial, cg->scal, kRR, 0);
+  } else {
+    k_update_r<T><<<nb, kThreads, 0, s>>>(cg->n, cg->scal, r, Ap, cg->partial);
+    k_finish<<<1, kThreads, 0, s>>>(nb, cg->partial, cg->scal, kRR, 2);
+  }
+  k_update_px<T><<<nb, kThreads, 0, s>>>(cg->n, cg->scal, x, p, z);
   MI355_LAUNCH_CHECK();
   return CEED_MI355_SUCCESS;
 }
@@ -185,6 +235,26 @@ int true_residual(CeedMi355CG cg, const double *b, const double *x, double *r64,
   return CEED_MI355_SUCCESS;
 }
 
+// With r in place: [z = invd r], optionally p = z, then r . r into kRR and r . z into kRho.
+template <typename T>
+int start_search(CeedMi355CG cg, bool set_p) {
+  hipStream_t s  = cg->stream;
+  const int   nb = nblocks(cg->n);
+  const T    *r  = (const T *)cg->r;
+  const T    *z  = r;
+  if (cg->jacobi) {
+    k_scale<T><<<nb, kThreads, 0, s>>>(cg->n, (const T *)cg->invd, r, (T *)cg->z);
+    z = (const T *)cg->z;
+  }
+  if (set_p) k_copy2<T, T><<<nb, kThreads, 0, s>>>(cg->n, z, (T *)cg->p, nullptr);
+  k_dot<T, T><<<nb, kThreads, 0, s>>>(cg->n, r, r, cg->partial);
+  k_finish<<<1, kThreads, 0, s>>>(nb, cg->partial, cg->scal, kRR, 0);
+  if (cg->jacobi) k_dot<T, T><<<nb, kThreads, 0, s>>>(cg->n, r, z, cg->partial);
+  k_finish<<<1, kThreads, 0, s>>>(nb, cg->partial, cg->scal, kRho, 0);
+  MI355_LAUNCH_CHECK();
+  return CEED_MI355_SUCCESS;
+}
+
 // mixed: fold the FP32 increment into x, replace the residual by the FP64 one, restart nothing
 int reliable_update(CeedMi355CG cg, const double *b, double *x) {
   hipStream_t s  = cg->stream;
@@ -194,10 +264,7 @@ int reliable_update(CeedMi355CG cg, const double *b, double *x) {
   int err = true_residual(cg, b, x, cg->r64, cg->Ap64);
   if (err) return err;
   k_copy2<double, float><<<nb, kThreads, 0, s>>>(cg->n, cg->r64, (float *)cg->r, nullptr);
-  k_dot<float, float><<<nb, kThreads, 0, s>>>(cg->n, (const float *)cg->r, (const float *)cg->r, cg->partial);
-  k_finish<<<1, kThreads, 0, s>>>(nb, cg->partial, cg->scal, kRR, 0);
-  MI355_LAUNCH_CHECK();
-  return CEED_MI355_SUCCESS;
+  return start_search<float>(cg, false);
 }
 
 // `len` iterations (plus, for mixed precision, the reliable update that ends them)
@@ -257,7 +324,8 @@ extern "C" int CeedMi355CGCreate(CeedMi355Operator op, CeedMi355Operator op64, C
   bool         ok = hipStreamCreateWithFlags(&cg->stream, hipStreamNonBlocking) == hipSuccess && hipEventCreateWithFlags(&cg->ev, hipEventDisableTiming) == hipSuccess &&
             hipMalloc(&cg->scal, sizeof(double) * kNumScal) == hipSuccess &&
             hipHostMalloc(&cg->h_scal, sizeof(double) * kNumScal, hipHostMallocDefault) == hipSuccess &&
-            hipMalloc(&cg->partial, sizeof(double) * kBlocks) == hipSuccess && hipMalloc(&cg->r, s * n) == hipSuccess &&
+            hipMalloc(&cg->partial, sizeof(double) * kBlocks) == hipSuccess && hipMalloc(&cg->partial2, sizeof(double) * kBlocks) == hipSuccess &&
+            hipMalloc(&cg->r, s * n) == hipSuccess &&
             hipMalloc(&cg->p, s * n) == hipSuccess && hipMalloc(&cg->Ap, s * n) == hipSuccess;
   if (ok && mixed)
     ok = hipMalloc(&cg->dx, sizeof(float) * n) == hipSuccess && hipMalloc(&cg->r64, sizeof(double) * n) == hipSuccess &&
@@ -267,6 +335,31 @@ extern "C" int CeedMi355CGCreate(CeedMi355Operator op, CeedMi355Operator op64, C
     MI355_CHECK(false, CEED_MI355_ERROR_BACKEND, "CG work-space allocation failed (%lld entries)", (long long)n);
   }
   *cg_out = cg;
+  return CEED_MI355_SUCCESS;
+}
+
+extern "C" int CeedMi355CGSetJacobi(CeedMi355CG cg, const double *diag, void *stream) {
+  MI355_CHECK(cg, CEED_MI355_ERROR_MAJOR, "null CG");
+  if (cg->graph) hipGraphExecDestroy(cg->graph);  // captured with the other recurrences
+  cg->graph = nullptr;
+  if (!diag) {
+    cg->jacobi = false;
+    return CEED_MI355_SUCCESS;
+  }
+  const size_t s = cg->mixed ? sizeof(float) : sizeof(double), n = (size_t)cg->n + 1;
+  if (!cg->invd) {
+    MI355_HIP(hipMalloc(&cg->invd, s * n));
+    MI355_HIP(hipMalloc(&cg->z, s * n));
+  }
+  MI355_HIP(hipEventRecord(cg->ev, (hipStream_t)stream));
+  MI355_HIP(hipStreamWaitEvent(cg->stream, cg->ev, 0));
+  const int nb = nblocks(cg->n);
+  if (cg->mixed) k_invert<float><<<nb, kThreads, 0, cg->stream>>>(cg->n, diag, (float *)cg->invd);
+  else k_invert<double><<<nb, kThreads, 0, cg->stream>>>(cg->n, diag, (double *)cg->invd);
+  MI355_LAUNCH_CHECK();
+  MI355_HIP(hipEventRecord(cg->ev, cg->stream));
+  MI355_HIP(hipStreamWaitEvent((hipStream_t)stream, cg->ev, 0));
+  cg->jacobi = true;
   return CEED_MI355_SUCCESS;
 }
 
@@ -286,17 +379,16 @@ extern "C" int CeedMi355CGSolve(CeedMi355CG cg, const double *b, double *x, int3
   k_finish<<<1, kThreads, 0, s>>>(nb, cg->partial, cg->scal, kBB, 0);
   MI355_LAUNCH_CHECK();
   int err;
-  if (cg->mixed) {  // r64 = b - A x, r = p = (float) r64, dx = 0
+  if (cg->mixed) {  // r64 = b - A x, r = (float) r64, p = [D^-1] r, dx = 0
     if ((err = true_residual(cg, b, x, cg->r64, cg->Ap64))) return err;
-    k_copy2<double, float><<<nb, kThreads, 0, s>>>(cg->n, cg->r64, (float *)cg->r, (float *)cg->p);
+    k_copy2<double, float><<<nb, kThreads, 0, s>>>(cg->n, cg->r64, (float *)cg->r, nullptr);
     k_fill<float><<<nb, kThreads, 0, s>>>(cg->n, 0.0f, (float *)cg->dx);
-    k_dot<float, float><<<nb, kThreads, 0, s>>>(cg->n, (const float *)cg->r, (const float *)cg->r, cg->partial);
-  } else {  // r = b - A x, p = r
+    err = start_search<float>(cg, true);
+  } else {  // r = b - A x, p = [D^-1] r
     if ((err = true_residual(cg, b, x, (double *)cg->r, (double *)cg->Ap))) return err;
-    k_copy2<double, double><<<nb, kThreads, 0, s>>>(cg->n, (const double *)cg->r, (double *)cg->p, nullptr);
-    k_dot<double, double><<<nb, kThreads, 0, s>>>(cg->n, (const double *)cg->r, (const double *)cg->r, cg->partial);
+    err = start_search<double>(cg, true);
   }
-  k_finish<<<1, kThreads, 0, s>>>(nb, cg->partial, cg->scal, kRR, 0);
+  if (err) return err;
   MI355_LAUNCH_CHECK();
   if ((err = read_scalars(cg))) return err;
   const double bb   = cg->h_scal[kBB];
@@ -335,6 +427,9 @@ extern "C" int CeedMi355CGDestroy(CeedMi355CG *cg) {
   hipFree(c->scal);
   hipHostFree(c->h_scal);
   hipFree(c->partial);
+  hipFree(c->partial2);
+  hipFree(c->invd);
+  hipFree(c->z);
   hipFree(c->r);
   hipFree(c->p);
   hipFree(c->Ap);

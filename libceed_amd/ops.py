@@ -264,7 +264,8 @@ class CG:
     """Conjugate gradients on the GPU (CeedMi355CGCreate/Solve).
 
     CG(op64): plain FP64 CG.  CG(op32, op64): FP32 iteration with the residual
-    replaced by the FP64 one every ``check_every`` iterations.
+    replaced by the FP64 one every ``check_every`` iterations.  set_jacobi(diag)
+    preconditions either with the operator diagonal.
     """
 
     def __init__(self, op: HexOperator, op64: HexOperator | None = None):
@@ -272,6 +273,13 @@ class CG:
         h = C.c_void_p()
         call("CeedMi355CGCreate", op._h, op64._h if op64 is not None else None, C.byref(h))
         self._h = h
+
+    def set_jacobi(self, diag: torch.Tensor | None) -> None:
+        """Jacobi preconditioning with the operator diagonal ``diag`` (FP64, e.g. HexOperator.assemble_diagonal); None: off."""
+        if diag is not None:
+            _device_tensor(diag, torch.float64)
+        self._diag = diag  # kept alive until the inverse is formed
+        call("CeedMi355CGSetJacobi", self._h, _ptr(diag) if diag is not None else None, _stream())
 
     def solve(self, b: torch.Tensor, x: torch.Tensor, max_iter: int = 1000, rtol: float = 1e-10, check_every: int = 25,
               use_graph: bool = True) -> CGStats:
