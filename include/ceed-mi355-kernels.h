@@ -159,12 +159,14 @@ int CeedMi355OperatorApplyAdd(CeedMi355Operator op, const void *u, void *v, void
 /* v  = A u  (CeedOperatorApply, interface/ceed-operator.c:2271-2294). */
 int CeedMi355OperatorApply(CeedMi355Operator op, const void *u, void *v, void *stream);
 /* Elements per workgroup of the fused kernel (1 <= epb, epb * Q^2 <= 256
- * threads, LDS <= 64 KiB). */
+ * threads, LDS <= 160 KiB, the gfx950 workgroup limit). */
 int CeedMi355OperatorSetLaunch(CeedMi355Operator op, int32_t elems_per_block);
 /* Thread layout of the fused kernel: 0 = z-columns per thread, x and y
  * contractions read point by point from LDS; 1 = whole 1D lines per thread
  * for the x and y contractions (about a third of the LDS traffic).  Both give
- * identical results.  elems_per_block <= 0 picks the largest that fits. */
+ * identical results.  elems_per_block <= 0 picks the default.  No
+ * counterpart in the reference: its hip-shared/hip-gen backends fix the
+ * thread layout per (P, Q) (backends/hip-shared/ceed-hip-shared-basis.c). */
 int CeedMi355OperatorSetLayout(CeedMi355Operator op, int32_t layout, int32_t elems_per_block);
 /* Current thread layout and elements per workgroup (either pointer may be NULL). */
 int CeedMi355OperatorGetLaunch(CeedMi355Operator op, int32_t *layout, int32_t *elems_per_block);

@@ -25,7 +25,14 @@ ApplyFn pick_apply_f32(int P, int Q, int num_comp, int qf);
 // point by point from LDS (hex_apply_kernel), or whole 1D lines per thread
 // for the x/y contractions (hex_apply_lines_kernel).
 enum { kLayoutColumns = 0, kLayoutLines = 1 };
-// Elements per workgroup giving ~256 threads and at most 64 KiB of LDS.
+// LDS a gfx950 workgroup may allocate (the CU's whole 160 KiB), and the
+// budget the default launch keeps to (so that several workgroups share a CU).
+constexpr int kMaxLdsBytes = 160 * 1024, kDefaultLdsBytes = 64 * 1024;
+// Threads per workgroup the kernels are compiled for (__launch_bounds__(256);
+// 512 was measured: slower line kernels at p = 7, 8 and spills in the large
+// column instantiations).
+constexpr int max_threads(int) { return 256; }
+// Elements per workgroup giving ~256 threads (line-layout diffusion: ~128) and at most kDefaultLdsBytes of LDS.
 int default_epb(int P, int Q, int qf, int scalar_bytes, int layout);
 int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes, int layout);
 

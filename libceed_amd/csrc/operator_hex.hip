@@ -26,7 +26,7 @@ int default_epb(int P, int Q, int qf, int scalar_bytes, int layout) {
   // autotuner's choices on MI355X)
   int epb = (layout == kLayoutLines && qf != kQfMass) ? 128 / (Q * Q) : 256 / (Q * Q);
   if (epb < 1) epb = 1;
-  while (epb > 1 && lds_bytes(P, Q, qf, epb, scalar_bytes, layout) > 64 * 1024) epb--;
+  while (epb > 1 && lds_bytes(P, Q, qf, epb, scalar_bytes, layout) > kDefaultLdsBytes) epb--;
   return epb;
 }
 }  // namespace mi355
@@ -230,8 +230,8 @@ bool tuned_launch(const CeedMi355Operator op, int *layout, int *epb) {
 }
 
 bool admissible(int P, int Q, int qf, int scalar_bytes, int layout, int epb) {
-  return (layout == mi355::kLayoutColumns || layout == mi355::kLayoutLines) && epb >= 1 && epb * Q * Q <= 256 &&
-         mi355::lds_bytes(P, Q, qf, epb, scalar_bytes, layout) <= 64 * 1024;
+  return (layout == mi355::kLayoutColumns || layout == mi355::kLayoutLines) && epb >= 1 &&
+         epb * Q * Q <= mi355::max_threads(layout) && mi355::lds_bytes(P, Q, qf, epb, scalar_bytes, layout) <= mi355::kMaxLdsBytes;
 }
 
 bool admissible_epb(const CeedMi355Operator op, int epb) {
@@ -544,7 +544,7 @@ extern "C" int CeedMi355OperatorApply(CeedMi355Operator op, const void *u, void 
 extern "C" int CeedMi355OperatorSetLaunch(CeedMi355Operator op, int32_t elems_per_block) {
   MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
   MI355_CHECK(admissible_epb(op, elems_per_block), CEED_MI355_ERROR_BACKEND,
-              "%d elements per workgroup do not fit (at most 256 threads and 64 KiB of LDS for P=%d Q=%d)", elems_per_block, op->P, op->Q);
+              "%d elements per workgroup do not fit (at most 256 threads and 160 KiB of LDS for P=%d Q=%d)", elems_per_block, op->P, op->Q);
   op->epb = elems_per_block;
   return CEED_MI355_SUCCESS;
 }
@@ -554,7 +554,7 @@ extern "C" int CeedMi355OperatorSetLayout(CeedMi355Operator op, int32_t layout, 
   MI355_CHECK(layout == mi355::kLayoutColumns || layout == mi355::kLayoutLines, CEED_MI355_ERROR_BACKEND, "unknown thread layout %d", layout);
   const int epb = elems_per_block > 0 ? elems_per_block : mi355::default_epb(op->P, op->Q, op->qf, op->scalar_bytes, layout);
   MI355_CHECK(admissible(op->P, op->Q, op->qf, op->scalar_bytes, layout, epb), CEED_MI355_ERROR_BACKEND,
-              "%d elements per workgroup do not fit layout %d (at most 256 threads and 64 KiB of LDS for P=%d Q=%d)", epb, layout, op->P, op->Q);
+              "%d elements per workgroup do not fit layout %d (at most 256 threads and 160 KiB of LDS for P=%d Q=%d)", epb, layout, op->P, op->Q);
   op->layout = layout;
   op->epb    = epb;
   return CEED_MI355_SUCCESS;
@@ -590,7 +590,7 @@ extern "C" int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int
   float     bms  = 1e30f;
   int       err  = CEED_MI355_SUCCESS;
   for (int layout = mi355::kLayoutColumns; layout <= mi355::kLayoutLines && !err; layout++) {
-    for (int epb = 1; epb * op->Q * op->Q <= 256 && !err; epb++) {
+    for (int epb = 1; epb * op->Q * op->Q <= mi355::max_threads(layout) && !err; epb++) {
       if (!admissible(op->P, op->Q, op->qf, op->scalar_bytes, layout, epb)) continue;
       op->layout = layout;
       op->epb    = epb;
