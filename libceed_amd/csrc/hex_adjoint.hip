@@ -6,10 +6,11 @@
 // g is the derivative of lambda^T A(kappa) u with respect to kappa_q, so for a
 // loss L(A(kappa) u) with lambda = dL/d(Au):  dL/dkappa = g, dL/du = v.
 //
-// The thread layout is the forward kernel's (hex_fused.hpp): a Q x Q slab of
-// threads per element with the z column in registers; u and lambda are
-// interpolated and differentiated one after the other, the gradient of u is
-// kept in registers while lambda's is formed.
+// Two thread layouts, as the forward kernel: the column layout
+// (hex_fused.hpp: a Q x Q slab of threads per element with the z column in
+// registers; u and lambda are interpolated and differentiated one after the
+// other, the gradient of u kept in registers while lambda's is formed) and
+// the line layout (hex_lines.hpp), chosen by the plan's layout.
 #include "hex_plan.hpp"
 
 namespace mi355 {
@@ -187,23 +188,275 @@ __global__ void __launch_bounds__(256) hex_adjoint_kernel(int64_t elem_begin, in
   }
 }
 
+// The same adjoint in the line thread layout (hex_lines.hpp): x and y
+// contractions by whole 1D lines per thread, z contractions and the pointwise
+// work on the (x, y) columns.  Five slabs per element: grad u's x and y parts
+// stay in two of them while lambda is interpolated and differentiated through
+// the other three.
+template <int P, int Q>
+__global__ void __launch_bounds__(256) hex_adjoint_lines_kernel(int64_t elem_begin, int64_t elem_end, int scatter, const int32_t *__restrict__ offsets,
+                                                                const int32_t *__restrict__ perm, const double *__restrict__ q_data,
+                                                                const double *__restrict__ mats, const double *__restrict__ u,
+                                                                const double *__restrict__ lam, double *__restrict__ v,
+                                                                const int32_t *__restrict__ tpos, double *__restrict__ g) {
+  constexpr int  NT = Q * Q, SLAB = Q * Q * Q, P3 = P * P * P, Q3 = Q * Q * Q, QC = 7, NS = 5;
+  constexpr bool kRowWise = Q >= 7;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int epb = blockDim.x / NT;
+  const int le = threadIdx.x / NT, t = threadIdx.x - le * NT;
+  double   *S0 = reinterpret_cast<double *>(lds_raw) + (size_t)le * NS * SLAB;
+  double   *S1 = S0 + SLAB, *S2 = S1 + SLAB, *S3 = S2 + SLAB, *S4 = S3 + SLAB;
+  auto      at = [](int z, int y, int x) { return (z * Q + y) * Q + x; };
+
+  const int64_t e    = elem_begin + xcd_chunk(blockIdx.x, gridDim.x) * epb + le;
+  const bool    live = le < epb && e < elem_end;
+  const int     cx = t % Q, cy = t / Q, ny = t % P, nz = t / P;
+  const bool    xline = t < P * P;
+  const bool    node  = live && cx < P && cy < P;
+  int           off[P];
+  if (node) {
+#pragma unroll
+    for (int z = 0; z < P; z++) off[z] = offsets[e * P3 + (z * P + cy) * P + cx];
+  }
+  const double *qd = q_data + (live ? e : 0) * (int64_t)QC * Q3 + (cy * Q + cx);
+  double        sq[Q][QC];
+#pragma unroll
+  for (int qz = 0; qz < Q; qz++)
+#pragma unroll
+    for (int k = 0; k < QC; k++) sq[qz][k] = __builtin_nontemporal_load(&qd[k * Q3 + qz * Q * Q]);
+
+  // reference-space gradient of src at the quadrature points: x part into SX,
+  // y part into SY, z part in dz (this thread's column); Sa and Sb are
+  // scratch (SY may be Sa: the last phase reads only Sb)
+  auto gradient = [&](const double *__restrict__ src, double *SX, double *SY, double *Sa, double *Sb, double(&dz)[Q]) {
+    if (node) {
+#pragma unroll
+      for (int z = 0; z < P; z++) Sa[at(z, cy, cx)] = src[off[z] >> 1];
+    }
+    __syncthreads();
+    if (xline) {  // B_x on the x-lines of nodes
+      const auto gB = phase_ptr(mats);
+      double     in[P];
+#pragma unroll
+      for (int x = 0; x < P; x++) in[x] = Sa[at(nz, ny, x)];
+#pragma unroll
+      for (int qx = 0; qx < Q; qx++) {
+        const auto m = kRowWise ? phase_ptr(mats + qx * P) : gB + qx * P;
+        double     s = 0;
+#pragma unroll
+        for (int b = 0; b < P; b++) s += m[b] * in[b];
+        Sb[at(nz, ny, qx)] = s;
+      }
+    }
+    __syncthreads();
+    if (t < Q * P) {  // B_y on the y-lines (qx, z), z < P
+      const auto gB = phase_ptr(mats);
+      const int  qx = t % Q, z = t / Q;
+      double     in[P];
+#pragma unroll
+      for (int b = 0; b < P; b++) in[b] = Sb[at(z, b, qx)];
+#pragma unroll
+      for (int qy = 0; qy < Q; qy++) {
+        const auto m = kRowWise ? phase_ptr(mats + qy * P) : gB + qy * P;
+        double     s = 0;
+#pragma unroll
+        for (int b = 0; b < P; b++) s += m[b] * in[b];
+        Sa[at(z, qy, qx)] = s;
+      }
+    }
+    __syncthreads();
+    {  // B_z and D_z on the column; the values go to Sb for the x/y derivatives
+      const auto gB = phase_ptr(mats);
+      double     col[P], xq[Q];
+#pragma unroll
+      for (int b = 0; b < P; b++) col[b] = Sa[at(b, cy, cx)];
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) {
+        const auto m = kRowWise ? phase_ptr(mats + qz * P) : gB + qz * P;
+        double     s = 0;
+#pragma unroll
+        for (int b = 0; b < P; b++) s += m[b] * col[b];
+        xq[qz] = s;
+      }
+      const auto gD = phase_ptr(mats + Q * P);
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) {
+        const auto m = kRowWise ? phase_ptr(mats + Q * P + qz * Q) : gD + qz * Q;
+        double     s = 0;
+#pragma unroll
+        for (int c = 0; c < Q; c++) s += m[c] * xq[c];
+        dz[qz]              = s;
+        Sb[at(qz, cy, cx)] = xq[qz];
+      }
+    }
+    __syncthreads();
+    {  // D_x by x-lines (qy = a, qz) into SX, D_y by y-lines (qx = a, qz) into SY
+      const auto gD = phase_ptr(mats + Q * P);
+      const int  a = t % Q, qz = t / Q;
+      double     in[Q];
+#pragma unroll
+      for (int c = 0; c < Q; c++) in[c] = Sb[at(qz, a, c)];
+#pragma unroll
+      for (int qx = 0; qx < Q; qx++) {
+        const auto m = kRowWise ? phase_ptr(mats + Q * P + qx * Q) : gD + qx * Q;
+        double     s = 0;
+#pragma unroll
+        for (int c = 0; c < Q; c++) s += m[c] * in[c];
+        SX[at(qz, a, qx)] = s;
+      }
+#pragma unroll
+      for (int c = 0; c < Q; c++) in[c] = Sb[at(qz, c, a)];
+#pragma unroll
+      for (int qy = 0; qy < Q; qy++) {
+        const auto m = kRowWise ? phase_ptr(mats + Q * P + qy * Q) : gD + qy * Q;
+        double     s = 0;
+#pragma unroll
+        for (int c = 0; c < Q; c++) s += m[c] * in[c];
+        SY[at(qz, qy, a)] = s;
+      }
+    }
+    __syncthreads();
+  };
+
+  double dzu[Q], dzl[Q];
+  gradient(u, S2, S3, S0, S1, dzu);    // grad u: S2 (x), S3 (y), dzu (z)
+  gradient(lam, S4, S0, S0, S1, dzl);  // grad lambda: S4 (x), S0 (y), dzl (z)
+  // ---- pointwise: w = S grad lambda, g = w . grad u, then kappa w (in place on the column)
+  double vz[Q];
+#pragma unroll
+  for (int qz = 0; qz < Q; qz++) {
+    double wx, wy, wz;
+    poisson_regs<double>(sq[qz], S4[at(qz, cy, cx)], S0[at(qz, cy, cx)], dzl[qz], wx, wy, wz);
+    if (live) g[(int64_t)perm[e] * Q3 + qz * Q * Q + cy * Q + cx] = wx * S2[at(qz, cy, cx)] + wy * S3[at(qz, cy, cx)] + wz * dzu[qz];
+    const double k = live ? sq[qz][6] : 0.0;
+    S4[at(qz, cy, cx)] = wx * k;
+    S0[at(qz, cy, cx)] = wy * k;
+    vz[qz]             = wz * k;
+  }
+  __syncthreads();
+  {  // D_x^T by x-lines into S2, D_y^T by y-lines into S3
+    const auto gD = phase_ptr(mats + Q * P);
+    const int  a = t % Q, qz = t / Q;
+    double     in[Q];
+#pragma unroll
+    for (int c = 0; c < Q; c++) in[c] = S4[at(qz, a, c)];
+#pragma unroll
+    for (int qx = 0; qx < Q; qx++) {
+      const auto m = kRowWise ? phase_ptr(mats + Q * P + qx) : gD + qx;
+      double     s = 0;
+#pragma unroll
+      for (int c = 0; c < Q; c++) s += m[c * Q] * in[c];
+      S2[at(qz, a, qx)] = s;
+    }
+#pragma unroll
+    for (int c = 0; c < Q; c++) in[c] = S0[at(qz, c, a)];
+#pragma unroll
+    for (int qy = 0; qy < Q; qy++) {
+      const auto m = kRowWise ? phase_ptr(mats + Q * P + qy) : gD + qy;
+      double     s = 0;
+#pragma unroll
+      for (int c = 0; c < Q; c++) s += m[c * Q] * in[c];
+      S3[at(qz, qy, a)] = s;
+    }
+  }
+  __syncthreads();
+  {  // sum of the three transposes on the column, then B_z^T into S1
+    const auto gDz = phase_ptr(mats + Q * P);
+    double     w[Q];
+#pragma unroll
+    for (int qz = 0; qz < Q; qz++) {
+      const auto m = kRowWise ? phase_ptr(mats + Q * P + qz) : gDz + qz;
+      double     s = S2[at(qz, cy, cx)] + S3[at(qz, cy, cx)];
+#pragma unroll
+      for (int c = 0; c < Q; c++) s += m[c * Q] * vz[c];
+      w[qz] = s;
+    }
+    const auto gBz = phase_ptr(mats);
+#pragma unroll
+    for (int b = 0; b < P; b++) {
+      const auto m = kRowWise ? phase_ptr(mats + b) : gBz + b;
+      double     s = 0;
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) s += m[qz * P] * w[qz];
+      S1[at(b, cy, cx)] = s;
+    }
+  }
+  __syncthreads();
+  if (t < Q * P) {  // B_y^T on the y-lines (qx, z), z < P, into S0
+    const auto gB = phase_ptr(mats);
+    const int  qx = t % Q, z = t / Q;
+    double     in[Q];
+#pragma unroll
+    for (int a = 0; a < Q; a++) in[a] = S1[at(z, a, qx)];
+#pragma unroll
+    for (int y = 0; y < P; y++) {
+      const auto m = kRowWise ? phase_ptr(mats + y) : gB + y;
+      double     s = 0;
+#pragma unroll
+      for (int a = 0; a < Q; a++) s += m[a * P] * in[a];
+      S0[at(z, y, qx)] = s;
+    }
+  }
+  __syncthreads();
+  if (xline) {  // B_x^T on the x-lines of nodes, into S1
+    const auto gB = phase_ptr(mats);
+    double     in[Q];
+#pragma unroll
+    for (int a = 0; a < Q; a++) in[a] = S0[at(nz, ny, a)];
+#pragma unroll
+    for (int x = 0; x < P; x++) {
+      const auto m = kRowWise ? phase_ptr(mats + x) : gB + x;
+      double     s = 0;
+#pragma unroll
+      for (int a = 0; a < Q; a++) s += m[a * P] * in[a];
+      S1[at(nz, ny, x)] = s;
+    }
+  }
+  __syncthreads();
+  if (node) {  // E^T from the node columns
+    double s[P];
+#pragma unroll
+    for (int z = 0; z < P; z++) s[z] = S1[at(z, cy, cx)];
+    if (scatter == kScatterEvec) {
+#pragma unroll
+      for (int z = 0; z < P; z++) v[tpos[e * (int64_t)P3 + (z * P + cy) * P + cx]] = s[z];
+    } else if (scatter == kScatterAtomic) {
+#pragma unroll
+      for (int z = 0; z < P; z++) unsafeAtomicAdd(&v[off[z] >> 1], s[z]);
+    } else {  // colour-exclusive; first element to reach a node writes it
+#pragma unroll
+      for (int z = 0; z < P; z++) {
+        double *dst = &v[off[z] >> 1];
+        *dst        = (off[z] & 1) ? s[z] : *dst + s[z];
+      }
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) zero_fill(int64_t n, double *__restrict__ x) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = 0.0;
 }
 
 template <int P, int Q>
 int launch_adjoint(const CeedMi355Operator_private *op, const double *u, const double *lam, double *v, double *g, hipStream_t s) {
-  const int    nt  = Q * Q;
-  const int    epb = op->epb;
-  const size_t lds = sizeof(double) * (size_t)(Q * P + Q * Q + epb * 3 * Q * Q * Q);
-  const bool   tmap = op->scatter == 0;
+  const int    nt    = Q * Q;
+  const int    epb   = op->epb;
+  const bool   lines = op->layout == kLayoutLines;
+  const size_t lds   = sizeof(double) * (size_t)(lines ? epb * 5 * Q * Q * Q : Q * P + Q * Q + epb * 3 * Q * Q * Q);
+  const bool   tmap  = op->scatter == 0;
+  MI355_CHECK(lds <= (size_t)kMaxLdsBytes, CEED_MI355_ERROR_BACKEND, "adjoint needs %zu bytes of LDS at %d elements per workgroup", lds, epb);
   for (size_t c = 0; c + 1 < op->color_start.size(); c++) {
     const int64_t b = op->color_start[c], e = op->color_start[c + 1];
     if (e <= b) continue;
     const int64_t grid    = (e - b + epb - 1) / epb;
     const int     scatter = tmap ? kScatterEvec : (op->scatter == CEED_MI355_PLAN_ATOMIC ? kScatterAtomic : kScatterOverwrite);
-    hex_adjoint_kernel<P, Q><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, scatter, op->d_offsets, op->d_perm, (const double *)op->d_qdata,
-                                                                  (const double *)op->d_mats, u, lam, tmap ? (double *)op->d_evec : v, op->d_tpos, g);
+    if (lines)
+      hex_adjoint_lines_kernel<P, Q><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, scatter, op->d_offsets, op->d_perm, (const double *)op->d_qdata,
+                                                                          (const double *)op->d_mats, u, lam, tmap ? (double *)op->d_evec : v,
+                                                                          op->d_tpos, g);
+    else
+      hex_adjoint_kernel<P, Q><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, scatter, op->d_offsets, op->d_perm, (const double *)op->d_qdata,
+                                                                    (const double *)op->d_mats, u, lam, tmap ? (double *)op->d_evec : v, op->d_tpos, g);
   }
   MI355_LAUNCH_CHECK();
   if (tmap) return transpose_gather(op, v, 1, s);

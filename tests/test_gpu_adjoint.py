@@ -31,10 +31,12 @@ def problem(cuda):
     return m, P, Q, qd, kappa
 
 
+@pytest.mark.parametrize("layout", [ops.LAYOUT_LINES, ops.LAYOUT_COLUMNS])
 @pytest.mark.parametrize("scatter", ["tmap", "color", "atomic"])
-def test_forward_and_adjoint_vs_oracle(problem, scatter):
+def test_forward_and_adjoint_vs_oracle(problem, scatter, layout):
     m, P, Q, qd, kappa = problem
     op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION_KAPPA, m, Q=Q, scatter=scatter)
+    op.set_layout(layout)
     rng = np.random.default_rng(4)
     u, lam = rng.uniform(-1, 1, (2, m.num_nodes))
     kd, ud, ld = (torch.from_numpy(a).cuda() for a in (kappa, u, lam))
