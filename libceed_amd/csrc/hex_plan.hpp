@@ -32,7 +32,8 @@ constexpr int kMaxLdsBytes = 160 * 1024, kDefaultLdsBytes = 64 * 1024;
 // 512 was measured: slower line kernels at p = 7, 8 and spills in the large
 // column instantiations).
 constexpr int max_threads(int) { return 256; }
-// Elements per workgroup giving ~256 threads (line-layout diffusion: ~128) and at most kDefaultLdsBytes of LDS.
+// Default elements per workgroup: for line-layout diffusion a table of measured
+// choices per Q, otherwise as many as fit 256 threads and kDefaultLdsBytes of LDS.
 int default_epb(int P, int Q, int qf, int scalar_bytes, int layout);
 int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes, int layout);
 

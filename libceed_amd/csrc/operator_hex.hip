@@ -21,10 +21,16 @@ int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes, int layout) {
 }
 
 int default_epb(int P, int Q, int qf, int scalar_bytes, int layout) {
-  // as many elements as fit 256 threads; diffusion in the line layout about
-  // two waves instead (its thirteen barriers wait on fewer waves; near the
-  // autotuner's choices on MI355X)
-  int epb = (layout == kLayoutLines && qf != kQfMass) ? 128 / (Q * Q) : 256 / (Q * Q);
+  if (layout == kLayoutLines && qf != kQfMass && Q >= 2 && Q <= 10) {
+    // diffusion in the line layout: the autotuner's choices for FP64 at ~8M
+    // DoFs on MI355X (Q = 2..10); within a few percent of the best for FP32
+    static const int table[11] = {0, 0, 8, 6, 4, 2, 3, 1, 3, 3, 2};
+    int              epb = table[Q];
+    while (epb > 1 && (epb * Q * Q > 256 || lds_bytes(P, Q, qf, epb, scalar_bytes, layout) > kMaxLdsBytes)) epb--;
+    return epb;
+  }
+  // as many elements as fit 256 threads and the default LDS budget
+  int epb = 256 / (Q * Q);
   if (epb < 1) epb = 1;
   while (epb > 1 && lds_bytes(P, Q, qf, epb, scalar_bytes, layout) > kDefaultLdsBytes) epb--;
   return epb;
