@@ -89,7 +89,7 @@ def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0,
     s = 8 if dtype == torch.float64 else 4
     B = alg_bytes(m.num_nodes, m.num_elem, P, Q, ncomp=num_comp, s=s)
     mode = {"tmap": "transpose map", "color": "element colours", "atomic": "atomics"}.get(
-        scatter, "transpose map" if P >= 6 else "element colours")
+        scatter, "element colours" if len(op.colors()) > 1 else "transpose map")
     F = alg_flops(m.num_elem, P, Q, ncomp=num_comp)
     kname = "hex_apply_lines_kernel" if layout == 1 else "hex_apply_kernel"
     res = {

@@ -319,12 +319,13 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
   op->comp_stride = comp_stride;
   op->l_size = l_size;
   op->apply = fn;
-  // E^T mode; by default the transpose map from P = 6 (its E-vector traffic,
-  // (P/p)^3 entries per node, pays off there; measured on MI355X) and colours below
+  // E^T mode; by default the transpose map from P = 6 in FP64 and P = 7 in
+  // FP32 (its E-vector and index traffic, (P/p)^3 entries per node, costs
+  // less than the colour launches from there; measured on MI355X), colours below
   if (flags & CEED_MI355_PLAN_ATOMIC) op->scatter = CEED_MI355_PLAN_ATOMIC;
   else if (flags & CEED_MI355_PLAN_COLOR) op->scatter = CEED_MI355_PLAN_COLOR;
   else if (flags & CEED_MI355_PLAN_TMAP) op->scatter = 0;
-  else op->scatter = P_1d >= 6 ? 0 : CEED_MI355_PLAN_COLOR;
+  else op->scatter = P_1d >= (scalar_type == CEED_MI355_FP32 ? 7 : 6) ? 0 : CEED_MI355_PLAN_COLOR;
   // launch: the tuned (layout, elements per workgroup) of this kind of plan if the
   // autotuner has run on one in this process, else the line layout (faster
   // than the column layout at every order measured on MI355X, equal at p <= 2)
