@@ -245,14 +245,16 @@ def run_adjoint(dofs: float = 4e6, p: int = 4, seed: int = 0, directions: int = 
             "fd_max_rel_error": {f"h={k:g}": v for k, v in worst.items()}}
 
 
-def run_drop_in(p: int = 4, n: int = 50, reps: int = 50, num_comp: int = 1):
+def run_drop_in(p: int = 4, n: int = 50, reps: int = 50, num_comp: int = 1, user: bool = False, fuse: bool = True):
     """CeedOperatorApply through the libCEED C API on /gpu/hip/mi355 (tests/c/bp-apply.c: gallery Poisson3DBuild +
-    Poisson3DApply, library defaults, no tuning), in a child process."""
+    Poisson3DApply, or the user QFunction of ex-hex compiled at run time, fused or staged; library defaults, no
+    tuning), in a child process."""
     exe = ROOT / "tests" / "c" / "bin" / "bp-apply"
     if not exe.exists():
         return None
-    res = subprocess.run([str(exe), "/gpu/hip/mi355", str(p), str(n), str(reps), str(num_comp)], capture_output=True, text=True,
-                         timeout=600)
+    env = dict(os.environ, CEED_MI355_FUSE_QFUNCTIONS="1" if fuse else "0")
+    res = subprocess.run([str(exe), "/gpu/hip/mi355", str(p), str(n), str(reps), str(num_comp)] + (["user"] if user else []),
+                         capture_output=True, text=True, timeout=600, env=env)
     if res.returncode != 0:
         return {"error": res.stderr.strip()[-300:]}
     return json.loads(res.stdout.strip().splitlines()[-1])
@@ -441,6 +443,10 @@ def main() -> None:
         for nc in (1, 3):
             drop_in[f"p4_n50_ncomp{nc}"] = run_drop_in(num_comp=nc)
             print(f"# drop-in C API: {drop_in[f'p4_n50_ncomp{nc}']}", file=sys.stderr, flush=True)
+        for fuse in (True, False):
+            key = f"p4_n50_user_{'fused' if fuse else 'staged'}"
+            drop_in[key] = run_drop_in(user=True, fuse=fuse)
+            print(f"# drop-in C API: {drop_in[key]}", file=sys.stderr, flush=True)
     adjoint = None
     if not args.no_sweep:
         adjoint = run_adjoint()

@@ -44,6 +44,10 @@ enum { CEED_MI355_FP64 = 0, CEED_MI355_FP32 = 1 };
  * MassApply / Vector3MassApply, Poisson3DApply / Vector3Poisson3DApply, and a
  * kappa-weighted Poisson3DApply for -div(kappa grad u)). */
 enum { CEED_MI355_QF_MASS = 0, CEED_MI355_QF_DIFFUSION = 1, CEED_MI355_QF_DIFFUSION_KAPPA = 2 };
+/* Field kinds of a run-time compiled fused operator (CeedMi355OperatorCreateHexJit):
+ * an active field evaluated with CEED_EVAL_INTERP or CEED_EVAL_GRAD, or the
+ * passive CEED_EVAL_NONE input (the plan's q_data). */
+enum { CEED_MI355_FIELD_INTERP = 0, CEED_MI355_FIELD_GRAD = 1, CEED_MI355_FIELD_PASSIVE = 2 };
 
 /* E^T mode flags of CeedMi355OperatorCreateHex (0: transpose map for P >= 6, colours below). */
 enum { CEED_MI355_PLAN_COLOR = 2, CEED_MI355_PLAN_ATOMIC = 4, CEED_MI355_PLAN_TMAP = 8 };
@@ -125,6 +129,22 @@ typedef struct CeedMi355Operator_private *CeedMi355Operator;
 int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_comp, int32_t num_elem, int32_t P_1d, int32_t Q_1d,
                                const double *interp_1d_host, const double *grad_1d_host, const int32_t *offsets_host, int32_t comp_stride,
                                int64_t l_size, int flags, CeedMi355Operator *op);
+/* A fused plan like CeedMi355OperatorCreateHex whose pointwise operator is a
+ * user QFunction: the source file (CEED_QFUNCTION(function_name), local
+ * includes inlined) is compiled with hipRTC into the line-layout fused kernel,
+ * which hands the QFunction its fields in their order (in_kinds / out_kinds:
+ * CEED_MI355_FIELD_*) one point at a time.  Scalar active fields (INTERP
+ * and/or GRAD, in and out), at most one passive input of q_comp values per
+ * point (set with CeedMi355OperatorSetQData), FP64.  The counterpart of the
+ * reference's hip-gen operator, which generates a fused kernel around any
+ * QFunction (backends/hip-gen/ceed-hip-gen-operator-build.cpp). */
+int CeedMi355OperatorCreateHexJit(const char *source_path, const char *function_name, int32_t num_in, const int32_t *in_kinds_host,
+                                  int32_t num_out, const int32_t *out_kinds_host, int32_t q_comp, int32_t num_elem, int32_t P_1d, int32_t Q_1d,
+                                  const double *interp_1d_host, const double *grad_1d_host, const int32_t *offsets_host, int64_t l_size, int flags,
+                                  CeedMi355Operator *op);
+/* Device pointer of the QFunction context for later applies of a plan made by
+ * CeedMi355OperatorCreateHexJit (NULL: none). */
+int CeedMi355OperatorSetContext(CeedMi355Operator op, void *ctx_device);
 /* Geometric factors from mesh coordinates on the GPU, FP64 arithmetic:
  * Mass3DBuild (w det J) or Poisson3DBuild (6 Voigt entries of
  * w det J J^-1 J^-T).  The mesh restriction has mesh_P^3 nodes per element,

@@ -31,6 +31,9 @@ SIGNATURES = {
     "CeedMi355BasisApply": (c_int, [vp, c_i32, c_int, c_int, c_int, vp, vp, vp]),
     "CeedMi355BasisDestroy": (c_int, [C.POINTER(vp)]),
     "CeedMi355OperatorCreateHex": (c_int, [c_int, c_int, c_i32, c_i32, c_i32, c_i32, vp, vp, vp, c_i32, c_i64, c_int, C.POINTER(vp)]),
+    "CeedMi355OperatorCreateHexJit": (c_int, [C.c_char_p, C.c_char_p, c_i32, vp, c_i32, vp, c_i32, c_i32, c_i32, c_i32, vp, vp, vp, c_i64, c_int,
+                                              C.POINTER(vp)]),
+    "CeedMi355OperatorSetContext": (c_int, [vp, vp]),
     "CeedMi355OperatorBuildQData": (c_int, [vp, c_int, c_i32, vp, vp, vp, vp, c_i32, vp, vp]),
     "CeedMi355OperatorSetQData": (c_int, [vp, vp, vp]),
     "CeedMi355OperatorGetQData": (c_int, [vp, vp, vp]),

@@ -118,6 +118,8 @@ DiagFn pick_diag(int P, int Q) {
 
 extern "C" int CeedMi355OperatorAssembleDiagonal(CeedMi355Operator op, void *diag, int add, void *stream) {
   MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
+  MI355_CHECK(op->qf == mi355::kQfMass || op->qf == mi355::kQfDiffusion || op->qf == mi355::kQfDiffusionKappa, CEED_MI355_ERROR_UNSUPPORTED,
+              "diagonal assembly needs a mass or diffusion plan");
   if (op->num_elem == 0) return CEED_MI355_SUCCESS;
   auto fn = op->scalar_type == CEED_MI355_FP64 ? mi355::pick_diag<double>(op->P, op->Q) : mi355::pick_diag<float>(op->P, op->Q);
   MI355_CHECK(fn, CEED_MI355_ERROR_UNSUPPORTED, "no diagonal kernel for P=%d Q=%d", op->P, op->Q);

@@ -21,7 +21,8 @@
 
 namespace mi355 {
 
-enum { kQfMass = 0, kQfDiffusion = 1, kQfDiffusionKappa = 2 };
+// kQfJit: a user QFunction compiled at run time into the line-layout kernel (operator_jit.hip)
+enum { kQfMass = 0, kQfDiffusion = 1, kQfDiffusionKappa = 2, kQfJit = 3 };
 // E^T modes: colour-exclusive read-modify-write (ApplyAdd), the same with the
 // first-touch overwrite (Apply), hardware atomics (one launch, any order), or
 // store of each contribution into an E-vector sorted by L entry (one launch;

@@ -39,6 +39,17 @@ int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes, int layout);
 
 }  // namespace mi355
 
+namespace mi355 {
+// A plan of CeedMi355OperatorCreateHex with the given kernel launcher and
+// geometric-factor components per point (operator_hex.hip).
+int create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_elem, int32_t P_1d, int32_t Q_1d, const double *interp_1d_host,
+                const double *grad_1d_host, const int32_t *offsets_host, int32_t comp_stride, int64_t l_size, int flags, int q_comp, ApplyFn fn,
+                CeedMi355Operator_private **op_out);
+}  // namespace mi355
+
+struct CeedMi355JitKernel;  // run-time compiled fused kernel of a kQfJit plan (operator_jit.hip)
+void mi355_jit_kernel_destroy(CeedMi355JitKernel *k);
+
 struct CeedMi355Operator_private {
   int     qf, scalar_type, scalar_bytes, num_comp, P, Q, q_comp;
   int32_t num_elem, comp_stride;
@@ -64,6 +75,7 @@ struct CeedMi355Operator_private {
   void                *d_mats    = nullptr;  // B (Q x P) then D (Q x Q), plan scalar type
   void                *d_qdata   = nullptr;  // [elem][q_comp][Q^3], packed order
   mi355::ApplyFn       apply     = nullptr;
+  CeedMi355JitKernel  *jit       = nullptr;  // kQfJit plans: the compiled kernel and the QFunction context
 };
 
 namespace mi355 {

@@ -42,6 +42,7 @@ namespace {
 using mi355::kQfDiffusion;
 using mi355::kQfDiffusionKappa;
 using mi355::kQfMass;
+using mi355::kQfJit;
 
 template <typename T>
 __global__ void __launch_bounds__(256) fill_kernel(int64_t n, T value, T *__restrict__ x) {
@@ -236,7 +237,7 @@ bool tuned_launch(const CeedMi355Operator op, int *layout, int *epb) {
 }
 
 bool admissible(int P, int Q, int qf, int scalar_bytes, int layout, int epb) {
-  return (layout == mi355::kLayoutColumns || layout == mi355::kLayoutLines) && epb >= 1 &&
+  return ((layout == mi355::kLayoutColumns && qf != kQfJit) || layout == mi355::kLayoutLines) && epb >= 1 &&
          epb * Q * Q <= mi355::max_threads(layout) && mi355::lds_bytes(P, Q, qf, epb, scalar_bytes, layout) <= mi355::kMaxLdsBytes;
 }
 
@@ -306,7 +307,14 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
   mi355::ApplyFn fn = scalar_type == CEED_MI355_FP64 ? mi355::pick_apply_f64(P_1d, Q_1d, num_comp, qf)
                                                      : mi355::pick_apply_f32(P_1d, Q_1d, num_comp, qf);
   MI355_CHECK(fn, CEED_MI355_ERROR_UNSUPPORTED, "no fused kernel for P=%d Q=%d num_comp=%d qf=%d", P_1d, Q_1d, num_comp, qf);
+  return mi355::create_plan(qf, scalar_type, num_comp, num_elem, P_1d, Q_1d, interp_1d_host, grad_1d_host, offsets_host, comp_stride, l_size,
+                            flags, qf == kQfMass ? 1 : (qf == kQfDiffusion ? 6 : 7), fn, op_out);
+}
 
+int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_elem, int32_t P_1d, int32_t Q_1d, const double *interp_1d_host,
+                       const double *grad_1d_host, const int32_t *offsets_host, int32_t comp_stride, int64_t l_size, int flags, int q_comp,
+                       mi355::ApplyFn fn, CeedMi355Operator *op_out) {
+  *op_out = nullptr;
   CeedMi355Operator op = new CeedMi355Operator_private();
   op->qf = qf;
   op->scalar_type = scalar_type;
@@ -314,7 +322,7 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
   op->num_comp = num_comp;
   op->P = P_1d;
   op->Q = Q_1d;
-  op->q_comp = qf == kQfMass ? 1 : (qf == kQfDiffusion ? 6 : 7);
+  op->q_comp = q_comp;
   op->num_elem = num_elem;
   op->comp_stride = comp_stride;
   op->l_size = l_size;
@@ -338,8 +346,11 @@ extern "C" int CeedMi355OperatorCreateHex(int qf, int scalar_type, int32_t num_c
   }
   if (const char *env = getenv("CEED_MI355_LAYOUT")) {
     const int l = atoi(env);
-    if (l == mi355::kLayoutColumns || l == mi355::kLayoutLines) op->layout = l, op->epb = mi355::default_epb(P_1d, Q_1d, qf, op->scalar_bytes, l);
+    if ((l == mi355::kLayoutColumns && qf != kQfJit) || l == mi355::kLayoutLines)
+      op->layout = l, op->epb = mi355::default_epb(P_1d, Q_1d, qf, op->scalar_bytes, l);
   }
+  if (qf == kQfJit && op->layout != mi355::kLayoutLines)  // run-time compiled kernels exist in the line layout only
+    op->layout = mi355::kLayoutLines, op->epb = mi355::default_epb(P_1d, Q_1d, qf, op->scalar_bytes, op->layout);
   if (const char *env = getenv("CEED_MI355_EPB")) {
     const int v = atoi(env);
     if (admissible(P_1d, Q_1d, qf, op->scalar_bytes, op->layout, v)) op->epb = v;
@@ -433,8 +444,8 @@ extern "C" int CeedMi355OperatorBuildQData(CeedMi355Operator op, int build, int3
                                            const double *mesh_grad_1d_host, const double *q_weight_1d_host, const int32_t *mesh_offsets_host,
                                            int32_t mesh_comp_stride, const double *coords, void *stream) {
   MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
-  MI355_CHECK((build == 0 && op->qf == kQfMass) || (build == 1 && op->qf != kQfMass), CEED_MI355_ERROR_BACKEND,
-              "geometric factor kind %d does not match the operator", build);
+  MI355_CHECK((build == 0 && op->qf == kQfMass) || (build == 1 && (op->qf == kQfDiffusion || op->qf == kQfDiffusionKappa)),
+              CEED_MI355_ERROR_BACKEND, "geometric factor kind %d does not match the operator", build);
   MI355_CHECK(mesh_P_1d >= 2 && mesh_P_1d <= op->Q, CEED_MI355_ERROR_UNSUPPORTED, "mesh P_1d=%d must be in 2..Q=%d", mesh_P_1d, op->Q);
   hipStream_t s = (hipStream_t)stream;
   const int   Q = op->Q, PM = mesh_P_1d, PM3 = PM * PM * PM;
@@ -559,6 +570,8 @@ extern "C" int CeedMi355OperatorSetLaunch(CeedMi355Operator op, int32_t elems_pe
 extern "C" int CeedMi355OperatorSetLayout(CeedMi355Operator op, int32_t layout, int32_t elems_per_block) {
   MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
   MI355_CHECK(layout == mi355::kLayoutColumns || layout == mi355::kLayoutLines, CEED_MI355_ERROR_BACKEND, "unknown thread layout %d", layout);
+  MI355_CHECK(layout == mi355::kLayoutLines || op->qf != kQfJit, CEED_MI355_ERROR_UNSUPPORTED,
+              "plans of run-time compiled QFunctions have the line layout only");
   const int epb = elems_per_block > 0 ? elems_per_block : mi355::default_epb(op->P, op->Q, op->qf, op->scalar_bytes, layout);
   MI355_CHECK(admissible(op->P, op->Q, op->qf, op->scalar_bytes, layout, epb), CEED_MI355_ERROR_BACKEND,
               "%d elements per workgroup do not fit layout %d (at most 256 threads and 160 KiB of LDS for P=%d Q=%d)", epb, layout, op->P, op->Q);
@@ -639,6 +652,7 @@ extern "C" int CeedMi355OperatorGetColors(CeedMi355Operator op, int32_t *num_col
 
 extern "C" int CeedMi355OperatorDestroy(CeedMi355Operator *op) {
   if (!op || !*op) return CEED_MI355_SUCCESS;
+  if ((*op)->jit) mi355_jit_kernel_destroy((*op)->jit);
   hipFree((*op)->d_offsets);
   hipFree((*op)->d_perm);
   hipFree((*op)->d_mats);

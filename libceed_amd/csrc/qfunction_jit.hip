@@ -90,6 +90,43 @@ typedef double CeedScalar;
 
 }  // namespace
 
+int mi355_jit_user_source(const char *source_path, std::string &out) {
+  std::set<std::string> seen;
+  std::string           user;
+  int                   ierr = load_source(source_path, seen, user, 0);
+  if (ierr) return ierr;
+  out = std::string(kPrelude) + "\n#line 1 \"" + source_path + "\"\n" + user + "\n";
+  return CEED_MI355_SUCCESS;
+}
+
+int mi355_jit_build(const std::string &code, const char *what, const char *kernel_name, hipModule_t *module, hipFunction_t *fn) {
+  hiprtcProgram prog;
+  MI355_CHECK(hiprtcCreateProgram(&prog, code.c_str(), "ceed-mi355-jit.hip", 0, nullptr, nullptr) == HIPRTC_SUCCESS, CEED_MI355_ERROR_BACKEND,
+              "hiprtcCreateProgram failed");
+  const char  *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  hiprtcResult rc     = hiprtcCompileProgram(prog, 8, opts);
+  if (rc != HIPRTC_SUCCESS) {
+    size_t log_size = 0;
+    hiprtcGetProgramLogSize(prog, &log_size);
+    std::string log(log_size + 1, '\0');
+    hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    MI355_CHECK(false, CEED_MI355_ERROR_BACKEND, "hipRTC failed for %s: %s", what, log.c_str());
+  }
+  size_t code_size = 0;
+  hiprtcGetCodeSize(prog, &code_size);
+  std::vector<char> binary(code_size);
+  hiprtcGetCode(prog, binary.data());
+  hiprtcDestroyProgram(&prog);
+  *module = nullptr;
+  if (hipModuleLoadData(module, binary.data()) != hipSuccess || hipModuleGetFunction(fn, *module, kernel_name) != hipSuccess) {
+    if (*module) hipModuleUnload(*module);
+    *module = nullptr;
+    MI355_CHECK(false, CEED_MI355_ERROR_BACKEND, "loading the compiled kernel %s failed", what);
+  }
+  return CEED_MI355_SUCCESS;
+}
+
 extern "C" int CeedMi355QFunctionCompile(const char *source_path, const char *function_name, int32_t num_in, const int32_t *in_sizes,
                                          int32_t num_out, const int32_t *out_sizes, CeedMi355QFunction *qf_out) {
   *qf_out = nullptr;
@@ -125,8 +162,8 @@ extern "C" int CeedMi355QFunctionCompile(const char *source_path, const char *fu
   hiprtcProgram prog;
   MI355_CHECK(hiprtcCreateProgram(&prog, code.c_str(), "ceed-mi355-qfunction.hip", 0, nullptr, nullptr) == HIPRTC_SUCCESS,
               CEED_MI355_ERROR_BACKEND, "hiprtcCreateProgram failed");
-  const char  *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast"};
-  hiprtcResult rc     = hiprtcCompileProgram(prog, 4, opts);
+  const char  *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  hiprtcResult rc     = hiprtcCompileProgram(prog, 8, opts);
   if (rc != HIPRTC_SUCCESS) {
     size_t log_size = 0;
     hiprtcGetProgramLogSize(prog, &log_size);

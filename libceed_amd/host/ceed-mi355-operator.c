@@ -11,6 +11,15 @@
  * fused GPU backends (backends/hip-gen/ceed-hip-gen-operator.c:52-258),
  * rebuilt for gfx950.
  *
+ * Fused path, run-time compiled: an operator whose pointwise function is a
+ * user (or any non-inlined) QFunction, with scalar active fields on one
+ * standard restriction and one 3D tensor basis (inputs INTERP and/or GRAD,
+ * outputs INTERP and/or GRAD) and at most one passive EVAL_NONE input, runs
+ * as one fused kernel too: the QFunction source is compiled with hipRTC into
+ * the line-layout kernel (CeedMi355OperatorCreateHexJit), the role of the
+ * reference's hip-gen backend (backends/hip-gen/ceed-hip-gen-operator-build.cpp).
+ * CEED_MI355_FUSE_QFUNCTIONS=0 keeps such operators on the generic path.
+ *
  * Generic path (any other operator): the staged apply of the reference
  * backends (backends/hip-ref/ceed-hip-ref-operator.c:258-638 and
  * backends/ref/ceed-ref-operator.c:145-500): E-vectors for all elements,
@@ -24,12 +33,13 @@
 #include "ceed-mi355-backend.h"
 
 typedef struct {
-  bool     is_set_up, fused;
+  bool     is_set_up, fused, jit;
   /* fused path */
   CeedMi355Operator hex;
   CeedSize          l_size;
   uint64_t          qdata_state;
   bool              qdata_loaded;
+  CeedInt           qdata_field; /* input field of the geometric factors, -1: none */
   /* generic path */
   CeedInt  num_in, num_out, num_elem, Q;
   double  *e_in[CEED_FIELD_MAX], *q_in[CEED_FIELD_MAX];
@@ -159,7 +169,8 @@ static int try_fused(CeedOperator op, OperatorData *d) {
     CeedCallBackend(CeedElemRestrictionRestoreOffsets(ru, &offsets));
   }
 done:
-  d->fused = ok;
+  d->fused       = ok;
+  d->qdata_field = 1;
   CeedCallBackend(CeedElemRestrictionDestroy(&ru));
   CeedCallBackend(CeedElemRestrictionDestroy(&rv));
   CeedCallBackend(CeedElemRestrictionDestroy(&rq));
@@ -174,7 +185,115 @@ done:
   return CEED_ERROR_SUCCESS;
 }
 
-/* reload the geometric factors into the plan when their vector changed */
+/* Fused plan with a run-time compiled QFunction, when the operator has the
+ * supported shape; leaves d->fused false otherwise. */
+static int try_fused_jit(CeedOperator op, OperatorData *d) {
+  Ceed                 ceed;
+  CeedQFunction        qf;
+  CeedInt              num_in, num_out, q_comp = 0, qdata_field = -1, P = 0, Q = 0;
+  CeedOperatorField   *in, *out;
+  CeedQFunctionField  *qf_in, *qf_out;
+  const char          *path, *name;
+  CeedElemRestriction  ru = NULL;
+  CeedBasis            bu = NULL;
+  int32_t              in_kinds[3], out_kinds[2];
+  int                  ok = 1;
+  const char          *env = getenv("CEED_MI355_FUSE_QFUNCTIONS");
+
+  d->fused = false;
+  if (env && !strcmp(env, "0")) return CEED_ERROR_SUCCESS;
+  CeedCallBackend(CeedOperatorGetCeed(op, &ceed));
+  CeedCallBackend(CeedOperatorGetQFunction(op, &qf));
+  CeedCallBackend(CeedQFunctionGetSourcePath(qf, &path));
+  CeedCallBackend(CeedQFunctionGetKernelName(qf, &name));
+  CeedCallBackend(CeedOperatorGetFields(op, &num_in, &in, &num_out, &out));
+  CeedCallBackend(CeedQFunctionGetFields(qf, NULL, &qf_in, NULL, &qf_out));
+  if (!path || !name || num_in < 1 || num_in > 3 || num_out < 1 || num_out > 2) ok = 0;
+  for (CeedInt f = 0; ok && f < num_in + num_out; f++) {
+    const bool          is_in = f < num_in;
+    const CeedInt       i     = is_in ? f : f - num_in;
+    CeedElemRestriction rstr;
+    CeedBasis           basis;
+    CeedVector          vec;
+    CeedEvalMode        mode;
+    CeedInt             size;
+    CeedCallBackend(CeedOperatorFieldGetElemRestriction(is_in ? in[i] : out[i], &rstr));
+    CeedCallBackend(CeedOperatorFieldGetBasis(is_in ? in[i] : out[i], &basis));
+    CeedCallBackend(CeedOperatorFieldGetVector(is_in ? in[i] : out[i], &vec));
+    CeedCallBackend(CeedQFunctionFieldGetEvalMode(is_in ? qf_in[i] : qf_out[i], &mode));
+    CeedCallBackend(CeedQFunctionFieldGetSize(is_in ? qf_in[i] : qf_out[i], &size));
+    if (vec == CEED_VECTOR_ACTIVE) {
+      /* all active fields: the same restriction and basis, scalar */
+      if ((mode != CEED_EVAL_INTERP && mode != CEED_EVAL_GRAD) || (ru && (rstr != ru || basis != bu)) || size != (mode == CEED_EVAL_GRAD ? 3 : 1))
+        ok = 0;
+      if (ok && !ru) {
+        CeedCallBackend(CeedElemRestrictionReferenceCopy(rstr, &ru));
+        CeedCallBackend(CeedBasisReferenceCopy(basis, &bu));
+      }
+      const int32_t kind = mode == CEED_EVAL_GRAD ? CEED_MI355_FIELD_GRAD : CEED_MI355_FIELD_INTERP;
+      if (is_in) in_kinds[i] = kind;
+      else out_kinds[i] = kind;
+    } else if (is_in && vec != CEED_VECTOR_NONE && mode == CEED_EVAL_NONE && qdata_field < 0 && rstr != CEED_ELEMRESTRICTION_NONE) {
+      in_kinds[i] = CEED_MI355_FIELD_PASSIVE;
+      qdata_field = i;
+      q_comp      = size;
+    } else {
+      ok = 0;
+    }
+    CeedCallBackend(CeedVectorDestroy(&vec));
+    CeedCallBackend(CeedBasisDestroy(&basis));
+    CeedCallBackend(CeedElemRestrictionDestroy(&rstr));
+  }
+  if (ok && ru) {
+    CeedRestrictionType type;
+    CeedInt             dim, nc, rc, elem_size, num_elem;
+    CeedCallBackend(CeedElemRestrictionGetType(ru, &type));
+    CeedCallBackend(CeedBasisGetDimension(bu, &dim));
+    CeedCallBackend(CeedBasisGetNumComponents(bu, &nc));
+    CeedCallBackend(CeedElemRestrictionGetNumComponents(ru, &rc));
+    CeedCallBackend(CeedBasisGetNumNodes1D(bu, &P));
+    CeedCallBackend(CeedBasisGetNumQuadraturePoints1D(bu, &Q));
+    CeedCallBackend(CeedElemRestrictionGetElementSize(ru, &elem_size));
+    CeedCallBackend(CeedElemRestrictionGetNumElements(ru, &num_elem));
+    ok = type == CEED_RESTRICTION_STANDARD && dim == 3 && nc == 1 && rc == 1 && elem_size == P * P * P && P >= 2 && P <= 9 && Q >= P && Q <= 10;
+    if (ok && qdata_field >= 0) {  /* the passive field: Q^3 points per element, its size per point */
+      CeedElemRestriction rq;
+      CeedInt             q_elem, q_nc, q_ne;
+      CeedCallBackend(CeedOperatorFieldGetElemRestriction(in[qdata_field], &rq));
+      CeedCallBackend(CeedElemRestrictionGetElementSize(rq, &q_elem));
+      CeedCallBackend(CeedElemRestrictionGetNumComponents(rq, &q_nc));
+      CeedCallBackend(CeedElemRestrictionGetNumElements(rq, &q_ne));
+      ok = q_elem == Q * Q * Q && q_nc == q_comp && q_ne == num_elem;
+      CeedCallBackend(CeedElemRestrictionDestroy(&rq));
+    }
+    if (ok) {
+      const CeedScalar *interp, *grad;
+      const CeedInt    *offsets;
+      CeedCallBackend(CeedBasisGetInterp1D(bu, &interp));
+      CeedCallBackend(CeedBasisGetGrad1D(bu, &grad));
+      CeedCallBackend(CeedElemRestrictionGetLVectorSize(ru, &d->l_size));
+      CeedCallBackend(CeedElemRestrictionGetOffsets(ru, CEED_MEM_HOST, &offsets));
+      int ierr = CeedMi355OperatorCreateHexJit(path, name, num_in, in_kinds, num_out, out_kinds, q_comp, num_elem, P, Q, interp, grad, offsets,
+                                               d->l_size, 0, &d->hex);
+      CeedCallBackend(CeedElemRestrictionRestoreOffsets(ru, &offsets));
+      if (ierr == CEED_MI355_ERROR_UNSUPPORTED) ok = 0;
+      else CeedCallBackend(CeedMi355Check(ceed, ierr));
+    }
+  } else {
+    ok = 0;
+  }
+  d->fused       = ok;
+  d->jit         = ok;
+  d->qdata_field = qdata_field;
+  CeedCallBackend(CeedElemRestrictionDestroy(&ru));
+  CeedCallBackend(CeedBasisDestroy(&bu));
+  CeedCallBackend(CeedQFunctionDestroy(&qf));
+  CeedCallBackend(CeedDestroy(&ceed));
+  return CEED_ERROR_SUCCESS;
+}
+
+/* reload the geometric factors into the plan when their vector changed, and
+ * (run-time compiled plans) hand over the current device context */
 static int fused_sync_qdata(CeedOperator op, OperatorData *d) {
   Ceed                ceed;
   CeedOperatorField  *in;
@@ -182,9 +301,18 @@ static int fused_sync_qdata(CeedOperator op, OperatorData *d) {
   CeedVector          vq;
   uint64_t            state;
   CeedCallBackend(CeedOperatorGetCeed(op, &ceed));
+  if (d->jit) {
+    CeedQFunction qf;
+    void         *ctx;
+    CeedCallBackend(CeedOperatorGetQFunction(op, &qf));
+    CeedCallBackend(CeedMi355QFunctionContextDevice(qf, &ctx));
+    CeedCallBackend(CeedMi355Check(ceed, CeedMi355OperatorSetContext(d->hex, ctx)));
+    CeedCallBackend(CeedQFunctionDestroy(&qf));
+  }
+  if (d->qdata_field < 0) return CeedDestroy(&ceed);
   CeedCallBackend(CeedOperatorGetFields(op, NULL, &in, NULL, NULL));
-  CeedCallBackend(CeedOperatorFieldGetElemRestriction(in[1], &rq));
-  CeedCallBackend(CeedOperatorFieldGetVector(in[1], &vq));
+  CeedCallBackend(CeedOperatorFieldGetElemRestriction(in[d->qdata_field], &rq));
+  CeedCallBackend(CeedOperatorFieldGetVector(in[d->qdata_field], &vq));
   CeedCallBackend(CeedVectorGetState(vq, &state));
   if (!d->qdata_loaded || state != d->qdata_state) {
     const double *e;
@@ -363,6 +491,7 @@ static int generic_apply_add(CeedOperator op, OperatorData *d, CeedVector in_vec
 static int setup(CeedOperator op, OperatorData *d) {
   if (d->is_set_up) return CEED_ERROR_SUCCESS;
   CeedCallBackend(try_fused(op, d));
+  if (!d->fused) CeedCallBackend(try_fused_jit(op, d));
   if (!d->fused) CeedCallBackend(generic_setup(op, d));
   d->is_set_up = true;
   return CEED_ERROR_SUCCESS;
@@ -419,7 +548,7 @@ static int assemble_diagonal(CeedOperator op, CeedVector assembled, bool add) {
   CeedCallBackend(CeedOperatorGetData(op, &d));
   CeedCallBackend(setup(op, d));
   CeedCallBackend(CeedOperatorGetCeed(op, &ceed));
-  CeedCheck(d->fused, ceed, CEED_ERROR_UNSUPPORTED, "diagonal assembly needs a gallery mass or diffusion operator on 3D tensor hexes");
+  CeedCheck(d->fused && !d->jit, ceed, CEED_ERROR_UNSUPPORTED, "diagonal assembly needs a gallery mass or diffusion operator on 3D tensor hexes");
   CeedCallBackend(CeedVectorGetArray(assembled, CEED_MEM_DEVICE, &da));
   CeedCallBackend(CeedMi355Check(ceed, CeedMi355OperatorAssembleDiagonal(d->hex, da, add, NULL)));
   CeedCallBackend(CeedVectorRestoreArray(assembled, &da));

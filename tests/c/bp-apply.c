@@ -7,13 +7,20 @@
  * Timing: CeedOperatorApply is asynchronous on the device; reading v on the
  * host waits for it.  Two runs of 5 and 5 + reps applies, each closed by a
  * host read, give the time per apply as their difference over reps.
- * Usage: bp-apply [resource] [p] [n] [reps] [ncomp]; prints one JSON line. */
+ * With a sixth argument "user", the scalar operator's pointwise function is
+ * the user QFunction of ex-hex instead of the gallery Poisson3DApply: the
+ * backend compiles it at run time into its fused kernel, or, with
+ * CEED_MI355_FUSE_QFUNCTIONS=0, into the staged path.
+ * Usage: bp-apply [resource] [p] [n] [reps] [ncomp] [user]; prints one JSON line. */
 #define _POSIX_C_SOURCE 199309L
 #include <ceed.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 #include <time.h>
+
+#include "ex-hex.h"
 
 static const double Amap[3][3] = {{1.0, 0.5, 0.0}, {0.0, 2.0, 0.25}, {0.0, 0.0, 3.0}};
 
@@ -37,7 +44,8 @@ static double run(CeedOperator op, CeedVector u, CeedVector v, int k, CeedInt n_
 int main(int argc, char **argv) {
   const char   *resource = argc > 1 ? argv[1] : "/gpu/hip/mi355";
   const CeedInt p = argc > 2 ? atoi(argv[2]) : 4, n = argc > 3 ? atoi(argv[3]) : 50, reps = argc > 4 ? atoi(argv[4]) : 50;
-  const CeedInt nc = argc > 5 ? atoi(argv[5]) : 1;
+  const CeedInt nc   = argc > 5 ? atoi(argv[5]) : 1;
+  const int     user = argc > 6 && !strcmp(argv[6], "user") && nc == 1;
   const CeedInt P = p + 1, Q = p + 2, E = n * n * n, nd = n * p + 1, N = nd * nd * nd;
   Ceed          ceed;
   if (CeedInit(resource, &ceed)) return 2;
@@ -86,7 +94,14 @@ int main(int argc, char **argv) {
   CeedOperatorSetField(op_build, "qdata", r_qd, CEED_BASIS_NONE, CEED_VECTOR_ACTIVE);
   CeedOperatorApply(op_build, x, qd, CEED_REQUEST_IMMEDIATE);
 
-  CeedQFunctionCreateInteriorByName(ceed, nc == 3 ? "Vector3Poisson3DApply" : "Poisson3DApply", &qf_apply);
+  if (user) {
+    CeedQFunctionCreateInterior(ceed, 1, ex_diffusion, ex_diffusion_loc, &qf_apply);
+    CeedQFunctionAddInput(qf_apply, "du", 3, CEED_EVAL_GRAD);
+    CeedQFunctionAddInput(qf_apply, "qdata", 6, CEED_EVAL_NONE);
+    CeedQFunctionAddOutput(qf_apply, "dv", 3, CEED_EVAL_GRAD);
+  } else {
+    CeedQFunctionCreateInteriorByName(ceed, nc == 3 ? "Vector3Poisson3DApply" : "Poisson3DApply", &qf_apply);
+  }
   CeedOperatorCreate(ceed, qf_apply, CEED_QFUNCTION_NONE, CEED_QFUNCTION_NONE, &op_apply);
   CeedOperatorSetField(op_apply, "du", r_u, b_u, CEED_VECTOR_ACTIVE);
   CeedOperatorSetField(op_apply, "qdata", r_qd, CEED_BASIS_NONE, qd);
@@ -106,8 +121,10 @@ int main(int argc, char **argv) {
   const double t_short = run(op_apply, u, v, 5, nc * N);
   const double t_long  = run(op_apply, u, v, 5 + reps, nc * N);
   const double ms      = 1e3 * (t_long - t_short) / reps;
-  printf("{\"resource\": \"%s\", \"p\": %d, \"n\": %d, \"num_comp\": %d, \"dofs\": %lld, \"reps\": %d, \"ms_per_apply\": %.5f, \"gdofs\": %.4f}\n",
-         resource, (int)p, (int)n, (int)nc, (long long)nc * N, (int)reps, ms, (double)nc * N / (ms * 1e-3) / 1e9);
+  printf("{\"resource\": \"%s\", \"p\": %d, \"n\": %d, \"num_comp\": %d, \"qfunction\": \"%s\", \"dofs\": %lld, \"reps\": %d, "
+         "\"ms_per_apply\": %.5f, \"gdofs\": %.4f}\n",
+         resource, (int)p, (int)n, (int)nc, !user ? "gallery" : (getenv("CEED_MI355_FUSE_QFUNCTIONS") && !strcmp(getenv("CEED_MI355_FUSE_QFUNCTIONS"), "0")) ? "user (staged)" : "user (run-time fused)", (long long)nc * N, (int)reps, ms,
+         (double)nc * N / (ms * 1e-3) / 1e9);
 
   CeedVectorDestroy(&x);
   CeedVectorDestroy(&qd);

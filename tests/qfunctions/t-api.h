@@ -1,7 +1,8 @@
 /*
  * User QFunctions for the API tests: they are not gallery functions, so
- * operators built from them take the backend's generic path (run-time
- * compiled QFunction between the standalone restriction and basis kernels).
+ * operators built from them are compiled at run time, either into the fused
+ * line-layout kernel (scalar active fields, at most one passive field) or as
+ * the pointwise stage between the standalone restriction and basis kernels.
  */
 #include <ceed/types.h>
 
@@ -49,6 +50,27 @@ CEED_QFUNCTION(user_mass_diffusion)(void *ctx, const CeedInt Q, const CeedScalar
   for (CeedInt i = 0; i < Q; i++) {
     const CeedScalar g0 = du[0 * Q + i], g1 = du[1 * Q + i], g2 = du[2 * Q + i];
     out[0][i]           = qm[i] * u[i];
+    out[1][0 * Q + i]   = g0 * qd[0 * Q + i] + g1 * qd[5 * Q + i] + g2 * qd[4 * Q + i];
+    out[1][1 * Q + i]   = g0 * qd[5 * Q + i] + g1 * qd[1 * Q + i] + g2 * qd[3 * Q + i];
+    out[1][2 * Q + i]   = g0 * qd[4 * Q + i] + g1 * qd[3 * Q + i] + g2 * qd[2 * Q + i];
+  }
+  return 0;
+}
+
+/* v = scale * qd * u, scale from the context */
+CEED_QFUNCTION(scaled_mass)(void *ctx, const CeedInt Q, const CeedScalar *const *in, CeedScalar *const *out) {
+  const struct ScaleContext *c = (const struct ScaleContext *)ctx;
+  for (CeedInt i = 0; i < Q; i++) out[0][i] = c->scale * in[1][i] * in[0][i];
+  return 0;
+}
+
+/* mass plus diffusion with one passive field of 7 values per point: the mass
+ * factor, then the 6 diffusion factors in Voigt order */
+CEED_QFUNCTION(user_mass_diffusion7)(void *ctx, const CeedInt Q, const CeedScalar *const *in, CeedScalar *const *out) {
+  const CeedScalar *u = in[0], *du = in[1], *q = in[2], *qd = in[2] + Q;
+  for (CeedInt i = 0; i < Q; i++) {
+    const CeedScalar g0 = du[0 * Q + i], g1 = du[1 * Q + i], g2 = du[2 * Q + i];
+    out[0][i]           = q[i] * u[i];
     out[1][0 * Q + i]   = g0 * qd[0 * Q + i] + g1 * qd[5 * Q + i] + g2 * qd[4 * Q + i];
     out[1][1 * Q + i]   = g0 * qd[5 * Q + i] + g1 * qd[1 * Q + i] + g2 * qd[3 * Q + i];
     out[1][2 * Q + i]   = g0 * qd[4 * Q + i] + g1 * qd[3 * Q + i] + g2 * qd[2 * Q + i];

@@ -252,13 +252,7 @@ def test_fused_operator_bitwise_repeatable(cuda):
     assert rel_l2(v1, ref) < RTOL
 
 
-@pytest.mark.parametrize("kind,p", [(0, 3), (1, 4), (1, 2)])
-def test_user_qfunction_generic_path(ceed, kind, p):
-    """A user QFunction (not in the gallery) runs the staged E -> B -> D -> B^T -> E^T path."""
-    m = hex_mesh(3, p, seed=20 + p)
-    P, Q = p + 1, p + 2
-    qdata, qd_rstr = build_qdata(ceed, m, Q, kind)
-    qref = oracle.hex_build_qdata(kind, m, Q)
+def _user_qf(ceed, kind):
     if kind == 0:
         qf = ceed.QFunction(1, None, f"{QF_FILE}:user_mass")
         qf.add_input("u", 1, libceed.EVAL_INTERP)
@@ -269,14 +263,55 @@ def test_user_qfunction_generic_path(ceed, kind, p):
         qf.add_input("du", 3, libceed.EVAL_GRAD)
         qf.add_input("qdata", 6, libceed.EVAL_NONE)
         qf.add_output("dv", 3, libceed.EVAL_GRAD)
-    u = np.random.default_rng(p).uniform(-1, 1, m.num_nodes)
-    v, _ = apply_operator(ceed, m, P, Q, kind, 1, qf, qdata, qd_rstr, u)
+    return qf
+
+
+@pytest.mark.parametrize("fuse", ["1", "0"])
+@pytest.mark.parametrize("kind,p", [(0, 3), (1, 4), (1, 2), (1, 5), (0, 7), (1, 8)])
+def test_user_qfunction(ceed, monkeypatch, fuse, kind, p):
+    """A user QFunction (not in the gallery): fused into the run-time compiled
+    line-layout kernel (default; colours below P = 6, the transpose map from
+    there), or the staged E -> B -> D -> B^T -> E^T path with
+    CEED_MI355_FUSE_QFUNCTIONS=0.  Apply and apply-add."""
+    monkeypatch.setenv("CEED_MI355_FUSE_QFUNCTIONS", fuse)
+    m = hex_mesh(3 if p < 7 else 2, p, seed=20 + p)
+    P, Q = p + 1, p + 2
+    qdata, qd_rstr = build_qdata(ceed, m, Q, kind)
+    qref = oracle.hex_build_qdata(kind, m, Q)
+    rng = np.random.default_rng(p)
+    u = rng.uniform(-1, 1, m.num_nodes)
+    v, _ = apply_operator(ceed, m, P, Q, kind, 1, _user_qf(ceed, kind), qdata, qd_rstr, u)
     ref = oracle.hex_apply(kind, 1, m.num_elem, P, Q, m.offsets, m.num_nodes, qref, u)
     assert rel_l2(v, ref) < RTOL
+    v0 = rng.uniform(-1, 1, u.size)
+    v2, _ = apply_operator(ceed, m, P, Q, kind, 1, _user_qf(ceed, kind), qdata, qd_rstr, u, v0=v0)
+    assert rel_l2(v2, ref + v0) < RTOL
 
 
-def test_two_field_operator(ceed):
-    """Two active inputs (INTERP and GRAD of u) and two active outputs: mass + diffusion."""
+def test_user_qfunction_fused_matches_staged(ceed, monkeypatch):
+    """The run-time fused kernel and the staged path agree to rounding on the
+    same user operator; the fused result repeats bit for bit."""
+    m = hex_mesh(4, 4, seed=31)
+    P, Q = 5, 6
+    qdata, qd_rstr = build_qdata(ceed, m, Q, 1)
+    u = np.random.default_rng(7).uniform(-1, 1, m.num_nodes)
+    out = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("CEED_MI355_FUSE_QFUNCTIONS", fuse)
+        out[fuse], op = apply_operator(ceed, m, P, Q, 1, 1, _user_qf(ceed, 1), qdata, qd_rstr, u)
+        uv, vv = ceed.Vector(m.num_nodes), ceed.Vector(m.num_nodes)
+        uv.set_array(u)
+        op.apply(uv, vv)
+        assert np.array_equal(vv.to_numpy(), out[fuse])
+    assert rel_l2(out["1"], out["0"]) < 1e-14
+
+
+@pytest.mark.parametrize("fuse", ["1", "0"])
+def test_two_field_operator(ceed, monkeypatch, fuse):
+    """Two active inputs (INTERP and GRAD of u), two passive inputs and two
+    active outputs: mass + diffusion.  Two passive inputs keep it on the
+    staged path whatever CEED_MI355_FUSE_QFUNCTIONS says."""
+    monkeypatch.setenv("CEED_MI355_FUSE_QFUNCTIONS", fuse)
     m = hex_mesh(3, 3, seed=5)
     P, Q = 4, 5
     N = m.num_nodes
@@ -305,6 +340,70 @@ def test_two_field_operator(ceed):
     ref = oracle.hex_apply(0, 1, m.num_elem, P, Q, m.offsets, N, oracle.hex_build_qdata(0, m, Q), u)
     ref = oracle.hex_apply(1, 1, m.num_elem, P, Q, m.offsets, N, oracle.hex_build_qdata(1, m, Q), u, v=ref)
     assert rel_l2(vv.to_numpy(), ref) < RTOL
+
+
+@pytest.mark.parametrize("fuse", ["1", "0"])
+def test_two_field_operator_one_passive(ceed, monkeypatch, fuse):
+    """INTERP and GRAD inputs and outputs with one passive field of 7 values
+    per point (mass factor, then the 6 diffusion factors): fused at run time
+    by default."""
+    monkeypatch.setenv("CEED_MI355_FUSE_QFUNCTIONS", fuse)
+    m = hex_mesh(3, 4, seed=6)
+    P, Q, N = 5, 6, m.num_nodes
+    qm, qd = oracle.hex_build_qdata(0, m, Q), oracle.hex_build_qdata(1, m, Q)
+    q3 = Q ** 3
+    both = np.concatenate([qm.reshape(m.num_elem, 1, q3), qd.reshape(m.num_elem, 6, q3)], axis=1).ravel()
+    q_rstr = ceed.StridedElemRestriction(m.num_elem, q3, 7, both.size, libceed.STRIDES_BACKEND)
+    qv = ceed.Vector(both.size)
+    qv.set_array(both)
+    qf = ceed.QFunction(1, None, f"{QF_FILE}:user_mass_diffusion7")
+    qf.add_input("u", 1, libceed.EVAL_INTERP)
+    qf.add_input("du", 3, libceed.EVAL_GRAD)
+    qf.add_input("q", 7, libceed.EVAL_NONE)
+    qf.add_output("v", 1, libceed.EVAL_INTERP)
+    qf.add_output("dv", 3, libceed.EVAL_GRAD)
+    basis = ceed.BasisTensorH1Lagrange(3, 1, P, Q, libceed.GAUSS)
+    rstr = ceed.ElemRestriction(m.num_elem, P ** 3, 1, N, N, m.offsets)
+    op = ceed.Operator(qf)
+    op.set_field("u", rstr, basis, libceed.VECTOR_ACTIVE)
+    op.set_field("du", rstr, basis, libceed.VECTOR_ACTIVE)
+    op.set_field("q", q_rstr, libceed.BASIS_NONE, qv)
+    op.set_field("v", rstr, basis, libceed.VECTOR_ACTIVE)
+    op.set_field("dv", rstr, basis, libceed.VECTOR_ACTIVE)
+    u = np.random.default_rng(2).uniform(-1, 1, N)
+    uv, vv = ceed.Vector(N), ceed.Vector(N)
+    uv.set_array(u)
+    op.apply(uv, vv)
+    ref = oracle.hex_apply(0, 1, m.num_elem, P, Q, m.offsets, N, qm, u)
+    ref = oracle.hex_apply(1, 1, m.num_elem, P, Q, m.offsets, N, qd, u, v=ref)
+    assert rel_l2(vv.to_numpy(), ref) < RTOL
+
+
+@pytest.mark.parametrize("fuse", ["1", "0"])
+def test_apply_qfunction_context(ceed, monkeypatch, fuse):
+    """An apply QFunction reading its context (v = scale * qd * u); a new
+    context value is seen by the next apply, fused or staged."""
+    monkeypatch.setenv("CEED_MI355_FUSE_QFUNCTIONS", fuse)
+    m = hex_mesh(3, 3, seed=12)
+    P, Q = 4, 5
+    qdata, qd_rstr = build_qdata(ceed, m, Q, 0)
+    qf = ceed.QFunction(1, None, f"{QF_FILE}:scaled_mass")
+    qf.add_input("u", 1, libceed.EVAL_INTERP)
+    qf.add_input("qdata", 1, libceed.EVAL_NONE)
+    qf.add_output("v", 1, libceed.EVAL_INTERP)
+    ctx = ceed.QFunctionContext()
+    ctx.set_data(np.array([3.0]))
+    qf.set_context(ctx)
+    u = np.random.default_rng(4).uniform(-1, 1, m.num_nodes)
+    v, op = apply_operator(ceed, m, P, Q, 0, 1, qf, qdata, qd_rstr, u)
+    ref = oracle.hex_apply(0, 1, m.num_elem, P, Q, m.offsets, m.num_nodes, oracle.hex_build_qdata(0, m, Q), u)
+    assert rel_l2(v, 3.0 * ref) < RTOL
+    with ctx.data() as d:
+        d[0] = -0.5
+    uv, vv = ceed.Vector(m.num_nodes), ceed.Vector(m.num_nodes)
+    uv.set_array(u)
+    op.apply(uv, vv)
+    assert rel_l2(vv.to_numpy(), -0.5 * ref) < RTOL
 
 
 def test_qfunction_context(ceed):
