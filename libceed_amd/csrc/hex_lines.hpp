@@ -22,7 +22,9 @@
 //   y-line (qx, z) = (t % Q, t / Q), t < Q*P              B_y, B_y^T
 //   x-line (qy, qz) and y-line (qx, qz) = (t % Q, t / Q)  the collocated derivatives
 // Slabs are [z][y][x] with strides (Q*Q, Q, 1); mass needs two per element,
-// diffusion four (the x and y derivative transposes land in separate slabs).
+// diffusion three (the y-derivative transpose waits in registers for a barrier
+// and then takes the slab the x-derivative transpose has read: one barrier more
+// than four slabs, and a quarter less LDS, so more workgroups fit a CU).
 #pragma once
 
 #include "hex_fused.hpp"
@@ -48,7 +50,7 @@ __device__ __forceinline__ const_ptr<T> phase_ptr(const T *p) {
 
 template <int QF>
 struct LineSlabs {
-  static constexpr int kSlabs = QF == kQfMass ? 2 : 4;
+  static constexpr int kSlabs = QF == kQfMass ? 2 : 3;
 };
 
 template <int P, int Q, int QF>
@@ -75,7 +77,6 @@ __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin
   T        *S0 = reinterpret_cast<T *>(lds_raw) + (size_t)le * NS * SLAB;
   T        *S1 = S0 + SLAB;
   T        *S2 = S1 + SLAB;  // diffusion only
-  T        *S3 = S2 + SLAB;  // diffusion only
   auto      at = [](int z, int y, int x) { return (z * Q + y) * Q + x; };
 
   const int64_t e    = elem_begin + xcd_chunk(blockIdx.x, gridDim.x) * epb + le;
@@ -229,31 +230,42 @@ __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin
         vz[qz]             = vzz;
       }
       __syncthreads();
-      // ---- D_x^T by x-lines into S1, D_y^T by y-lines into S3
+      // ---- D_x^T by x-lines into S1; D_y^T by y-lines, held in registers
+      // until every x-line has read S2, then written over it
       {
-        const auto gD = phase_ptr(mats + Q * P);
         const int a = t % Q, qz = t / Q;
-        T         in[Q];
+        T         ty[Q];
+        {
+          const auto gD = phase_ptr(mats + Q * P);
+          T          in[Q];
 #pragma unroll
-        for (int c = 0; c < Q; c++) in[c] = S2[at(qz, a, c)];
+          for (int c = 0; c < Q; c++) in[c] = S2[at(qz, a, c)];
 #pragma unroll
-        for (int qx = 0; qx < Q; qx++) {
-          const auto m = kRowWise ? phase_ptr(mats + Q * P + qx) : gD + qx;
-          T s = 0;
+          for (int qx = 0; qx < Q; qx++) {
+            const auto m = kRowWise ? phase_ptr(mats + Q * P + qx) : gD + qx;
+            T s = 0;
 #pragma unroll
-          for (int c = 0; c < Q; c++) s += m[c * Q] * in[c];
-          S1[at(qz, a, qx)] = s;
+            for (int c = 0; c < Q; c++) s += m[c * Q] * in[c];
+            S1[at(qz, a, qx)] = s;
+          }
         }
+        {
+          const auto gD = phase_ptr(mats + Q * P);
+          T          in[Q];
 #pragma unroll
-        for (int c = 0; c < Q; c++) in[c] = S0[at(qz, c, a)];
+          for (int c = 0; c < Q; c++) in[c] = S0[at(qz, c, a)];
 #pragma unroll
-        for (int qy = 0; qy < Q; qy++) {
-          const auto m = kRowWise ? phase_ptr(mats + Q * P + qy) : gD + qy;
-          T s = 0;
+          for (int qy = 0; qy < Q; qy++) {
+            const auto m = kRowWise ? phase_ptr(mats + Q * P + qy) : gD + qy;
+            T s = 0;
 #pragma unroll
-          for (int c = 0; c < Q; c++) s += m[c * Q] * in[c];
-          S3[at(qz, qy, a)] = s;
+            for (int c = 0; c < Q; c++) s += m[c * Q] * in[c];
+            ty[qy] = s;
+          }
         }
+        __syncthreads();
+#pragma unroll
+        for (int qy = 0; qy < Q; qy++) S2[at(qz, qy, a)] = ty[qy];
       }
       __syncthreads();
       // ---- sum of the three transposes on the column, then B_z^T into S0
@@ -262,7 +274,7 @@ __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin
 #pragma unroll
       for (int qz = 0; qz < Q; qz++) {
         const auto m = kRowWise ? phase_ptr(mats + Q * P + qz) : gDz + qz;
-        T s = S1[at(qz, cy, cx)] + S3[at(qz, cy, cx)];
+        T s = S1[at(qz, cy, cx)] + S2[at(qz, cy, cx)];
 #pragma unroll
         for (int c = 0; c < Q; c++) s += m[c * Q] * vz[c];
         w[qz] = s;

@@ -15,7 +15,7 @@ namespace mi355 {
 void collocated_grad(int P, int Q, const double *interp, const double *grad, double *collo);
 
 int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes, int layout) {
-  if (layout == kLayoutLines) return scalar_bytes * epb * (qf == kQfMass ? 2 : 4) * Q * Q * Q;
+  if (layout == kLayoutLines) return scalar_bytes * epb * (qf == kQfMass ? 2 : (qf == kQfJit ? 4 : 3)) * Q * Q * Q;
   const int slabs = qf == kQfMass ? 2 : 3;
   return scalar_bytes * (Q * P + Q * Q + epb * slabs * Q * Q * Q);
 }
@@ -24,7 +24,7 @@ int default_epb(int P, int Q, int qf, int scalar_bytes, int layout) {
   if (layout == kLayoutLines && qf != kQfMass && Q >= 2 && Q <= 10) {
     // diffusion in the line layout: the autotuner's choices for FP64 at ~8M
     // DoFs on MI355X (Q = 2..10); within a few percent of the best for FP32
-    static const int table[11] = {0, 0, 8, 6, 4, 2, 3, 1, 3, 3, 2};
+    static const int table[11] = {0, 0, 8, 6, 4, 2, 3, 1, 4, 3, 2};
     int              epb = table[Q];
     while (epb > 1 && (epb * Q * Q > 256 || lds_bytes(P, Q, qf, epb, scalar_bytes, layout) > kMaxLdsBytes)) epb--;
     return epb;
