@@ -15,7 +15,7 @@ namespace mi355 {
 void collocated_grad(int P, int Q, const double *interp, const double *grad, double *collo);
 
 int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes, int layout) {
-  if (layout == kLayoutLines) return scalar_bytes * epb * (qf == kQfMass ? 2 : (qf == kQfJit ? 4 : 3)) * Q * Q * Q;
+  if (layout == kLayoutLines) return scalar_bytes * epb * (qf == kQfMass ? 2 : 3) * Q * Q * Q;
   const int slabs = qf == kQfMass ? 2 : 3;
   return scalar_bytes * (Q * P + Q * Q + epb * slabs * Q * Q * Q);
 }

@@ -41,7 +41,7 @@ namespace {
 // The kernel body, compiled by hipRTC after the user's QFunction source and
 // the generated glue (mi355_point).  Template parameters: P, Q, which
 // active input/output evaluations the QFunction uses, QC passive values per
-// point.  Slabs [z][y][x] with strides (Q*Q, Q, 1), four per element.
+// point.  Slabs [z][y][x] with strides (Q*Q, Q, 1), three per element.
 const char *kBody = R"(
 template <typename T> using mi355_cptr = const __attribute__((address_space(4))) T *;
 template <typename T> __device__ __forceinline__ mi355_cptr<T> mi355_phase(const T *p) { asm volatile("" : "+s"(p)); return (mi355_cptr<T>)p; }
@@ -60,7 +60,7 @@ __device__ __forceinline__ void mi355_fused(long elem_begin, long elem_end, int 
   constexpr bool kRowWise = Q >= 7;
   const int epb = blockDim.x / NT;
   const int le = threadIdx.x / NT, t = threadIdx.x - le * NT;
-  double   *S0 = lds + (long)le * 4 * SLAB, *S1 = S0 + SLAB, *S2 = S1 + SLAB, *S3 = S2 + SLAB;
+  double   *S0 = lds + (long)le * 3 * SLAB, *S1 = S0 + SLAB, *S2 = S1 + SLAB;
   auto      at = [](int z, int y, int x) { return (z * Q + y) * Q + x; };
   const long e    = elem_begin + mi355_xcd_chunk(blockIdx.x, gridDim.x) * epb + le;
   const bool live = le < epb && e < elem_end && (!elem_sys || sys_active[elem_sys[e]]);
@@ -73,6 +73,16 @@ __device__ __forceinline__ void mi355_fused(long elem_begin, long elem_end, int 
     for (int z = 0; z < P; z++) off[z] = offsets[e * P3 + (z * P + cy) * P + cx];
   }
   const double *qd = q_data + (live ? e : 0) * (long)(QC > 0 ? QC : 1) * Q3 + (cy * Q + cx);
+  // up to 7 passive values per point: the column's values all loaded at the
+  // element start, in flight together while E and the interpolation run
+  constexpr bool kEarly = QC > 0 && QC <= 7;
+  double         sq[kEarly ? Q : 1][kEarly ? QC : 1];
+  if (kEarly) {
+#pragma unroll
+    for (int qz = 0; qz < Q; qz++)
+#pragma unroll
+      for (int k = 0; k < QC; k++) sq[qz][k] = __builtin_nontemporal_load(&qd[k * Q3 + qz * Q * Q]);
+  }
 
   // ---- E, then B_x, B_y by lines and B_z on the column
   if (node) {
@@ -178,7 +188,7 @@ __device__ __forceinline__ void mi355_fused(long elem_begin, long elem_end, int 
       fg[2] = dz[qz];
     }
 #pragma unroll
-    for (int k = 0; k < QC; k++) fq[k] = __builtin_nontemporal_load(&qd[k * Q3 + qz * Q * Q]);
+    for (int k = 0; k < QC; k++) fq[k] = kEarly ? sq[kEarly ? qz : 0][kEarly ? k : 0] : __builtin_nontemporal_load(&qd[k * Q3 + qz * Q * Q]);
     mi355_point(ctx, fu, fg, fq, ou, og);
     if (!live) ou[0] = og[0] = og[1] = og[2] = 0;
     wu[qz] = ou[0];
@@ -192,10 +202,10 @@ __device__ __forceinline__ void mi355_fused(long elem_begin, long elem_end, int 
   double w[Q];
   if (OUT_G) {
     __syncthreads();
-    {
+    {  // D_y^T waits in registers until every x-line has read S2, then takes S2
       const auto gD = mi355_phase(mats + Q * P);
       const int  a = t % Q, qz = t / Q;
-      double     in[Q];
+      double     in[Q], ty[Q];
 #pragma unroll
       for (int c = 0; c < Q; c++) in[c] = S2[at(qz, a, c)];
 #pragma unroll
@@ -214,15 +224,18 @@ __device__ __forceinline__ void mi355_fused(long elem_begin, long elem_end, int 
         double     s = 0;
 #pragma unroll
         for (int c = 0; c < Q; c++) s += m[c * Q] * in[c];
-        S3[at(qz, qy, a)] = s;
+        ty[qy] = s;
       }
+      __syncthreads();
+#pragma unroll
+      for (int qy = 0; qy < Q; qy++) S2[at(qz, qy, a)] = ty[qy];
     }
     __syncthreads();
     const auto gDz = mi355_phase(mats + Q * P);
 #pragma unroll
     for (int qz = 0; qz < Q; qz++) {
       const auto m = kRowWise ? mi355_phase(mats + Q * P + qz) : gDz + qz;
-      double     s = S1[at(qz, cy, cx)] + S3[at(qz, cy, cx)];
+      double     s = S1[at(qz, cy, cx)] + S2[at(qz, cy, cx)];
 #pragma unroll
       for (int c = 0; c < Q; c++) s += m[c * Q] * vz[c];
       w[qz] = OUT_U ? s + wu[qz] : s;
@@ -308,7 +321,7 @@ struct JitArgs {
 
 int launch_jit(const CeedMi355Operator_private *op, const void *u, void *v, int overwrite, hipStream_t s) {
   const int    Q = op->Q, nt = Q * Q, epb = op->epb;
-  const size_t lds  = sizeof(double) * (size_t)epb * 4 * Q * Q * Q;
+  const size_t lds  = sizeof(double) * (size_t)epb * 3 * Q * Q * Q;
   const bool   tmap = op->scatter == 0;
   for (size_t c = 0; c + 1 < op->color_start.size(); c++) {
     const int64_t b = op->color_start[c], e = op->color_start[c + 1];
