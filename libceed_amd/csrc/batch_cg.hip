@@ -156,6 +156,9 @@ struct CeedMi355BatchCG_private {
   int32_t                       *d_chunk_sys = nullptr, *d_sys_chunk = nullptr, *d_iters = nullptr, *h_iters = nullptr;
   int32_t                       *d_active = nullptr, *d_prev = nullptr;            // per system
   std::vector<int32_t *>         d_elem_sys;                                       // per plan: system of each packed element
+  std::vector<int64_t *>         d_op_chunk_begin;                                 // per plan: its L range in chunks of one system
+  std::vector<int32_t *>         d_op_chunk_sys;                                   //   (relative begins + end) and their systems
+  std::vector<int64_t>           op_num_chunks;
   double                        *d_scal = nullptr, *h_scal = nullptr, *d_partial = nullptr;
   double                        *r = nullptr, *p = nullptr, *Ap = nullptr;
   hipStream_t                    stream = nullptr;
@@ -173,11 +176,17 @@ namespace {
 int apply_all(CeedMi355BatchCG cg, const double *u, double *v, bool skip = false) {
   for (size_t i = 0; i < cg->ops.size(); i++) {
     CeedMi355Operator op = cg->ops[i];
-    op->d_elem_sys       = skip ? cg->d_elem_sys[i] : nullptr;
-    op->d_sys_active     = skip ? cg->d_active : nullptr;
-    const int err        = CeedMi355OperatorApply(op, u + cg->op_begin[i], v + cg->op_begin[i], cg->stream);
-    op->d_elem_sys       = nullptr;
-    op->d_sys_active     = nullptr;
+    op->d_elem_sys           = skip ? cg->d_elem_sys[i] : nullptr;
+    op->d_sys_active         = skip ? cg->d_active : nullptr;
+    op->d_gather_chunk_begin = skip ? cg->d_op_chunk_begin[i] : nullptr;
+    op->d_gather_chunk_sys   = skip ? cg->d_op_chunk_sys[i] : nullptr;
+    op->gather_num_chunks    = skip ? cg->op_num_chunks[i] : 0;
+    const int err            = CeedMi355OperatorApply(op, u + cg->op_begin[i], v + cg->op_begin[i], cg->stream);
+    op->d_elem_sys           = nullptr;
+    op->d_sys_active         = nullptr;
+    op->d_gather_chunk_begin = nullptr;
+    op->d_gather_chunk_sys   = nullptr;
+    op->gather_num_chunks    = 0;
     if (err) return err;
   }
   return CEED_MI355_SUCCESS;
@@ -315,6 +324,25 @@ extern "C" int CeedMi355BatchCGCreate(int32_t num_ops, const CeedMi355Operator *
     ok = ok && hipMalloc(&d, sizeof(int32_t) * sys.size()) == hipSuccess &&
          hipMemcpy(d, sys.data(), sizeof(int32_t) * sys.size(), hipMemcpyHostToDevice) == hipSuccess;
     cg->d_elem_sys.push_back(d);
+    // the plan's range [b, e) cut at the reduction chunks' bounds (a system may span plans)
+    const int64_t        b = op_begin_host[i], e = b + op->l_size;
+    std::vector<int64_t> cb;
+    std::vector<int32_t> cs;
+    for (size_t c = 0; c + 1 < chunk_begin.size(); c++) {
+      const int64_t lo = std::max(chunk_begin[c], b), hi = std::min(chunk_begin[c + 1], e);
+      if (lo >= hi) continue;
+      cb.push_back(lo - b);
+      cs.push_back(chunk_sys[c]);
+    }
+    cb.push_back(e - b);
+    int64_t *dcb = nullptr;
+    int32_t *dcs = nullptr;
+    ok = ok && hipMalloc(&dcb, sizeof(int64_t) * cb.size()) == hipSuccess && hipMalloc(&dcs, sizeof(int32_t) * (cs.size() + 1)) == hipSuccess &&
+         hipMemcpy(dcb, cb.data(), sizeof(int64_t) * cb.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         (cs.empty() || hipMemcpy(dcs, cs.data(), sizeof(int32_t) * cs.size(), hipMemcpyHostToDevice) == hipSuccess);
+    cg->d_op_chunk_begin.push_back(dcb);
+    cg->d_op_chunk_sys.push_back(dcs);
+    cg->op_num_chunks.push_back((int64_t)cs.size());
   }
   ok = ok && hipMemcpy(cg->d_chunk_begin, chunk_begin.data(), sizeof(int64_t) * chunk_begin.size(), hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(cg->d_chunk_sys, chunk_sys.data(), sizeof(int32_t) * chunk_sys.size(), hipMemcpyHostToDevice) == hipSuccess &&
@@ -398,6 +426,8 @@ extern "C" int CeedMi355BatchCGDestroy(CeedMi355BatchCG *cg) {
   hipFree(c->d_active);
   hipFree(c->d_prev);
   for (int32_t *d : c->d_elem_sys) hipFree(d);
+  for (int64_t *d : c->d_op_chunk_begin) hipFree(d);
+  for (int32_t *d : c->d_op_chunk_sys) hipFree(d);
   if (c->ev) hipEventDestroy(c->ev);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;

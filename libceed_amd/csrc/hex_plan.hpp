@@ -67,6 +67,12 @@ struct CeedMi355Operator_private {
   // set by a batched solver: system of each (packed) element, and the
   // solver's per-system active flags; elements of inactive systems are skipped
   const int32_t *d_elem_sys = nullptr, *d_sys_active = nullptr;
+  // set by a batched solver with the above: the plan's L range cut into chunks
+  // of one system each (begin offsets relative to the plan, the last entry the
+  // end); the transpose gather skips the chunks of inactive systems
+  const int64_t *d_gather_chunk_begin = nullptr;
+  const int32_t *d_gather_chunk_sys   = nullptr;
+  int64_t        gather_num_chunks    = 0;
   // colour c owns packed elements [color_start[c], color_start[c+1])
   std::vector<int64_t> color_start;
   std::vector<int32_t> perm;  // packed position -> caller's element index

@@ -284,10 +284,36 @@ __global__ void __launch_bounds__(256) transpose_gather_kernel(int64_t l_size, c
   }
 }
 
+// The same over chunks of L entries, one block per chunk; chunks of inactive
+// systems (batched solvers) are left out
+template <typename T>
+__global__ void __launch_bounds__(256) transpose_gather_chunks_kernel(const int64_t *__restrict__ chunk_begin, const int32_t *__restrict__ chunk_sys,
+                                                                      const int32_t *__restrict__ active, const int32_t *__restrict__ row,
+                                                                      const T *__restrict__ evec, T *__restrict__ v, int overwrite) {
+  const int64_t c = blockIdx.x;
+  if (!active[chunk_sys[c]]) return;  // uniform over the block
+  for (int64_t t = chunk_begin[c] + threadIdx.x; t < chunk_begin[c + 1]; t += blockDim.x) {
+    T acc = overwrite ? T(0) : v[t];
+    for (int32_t m = row[t]; m < row[t + 1]; m++) acc += evec[m];
+    v[t] = acc;
+  }
+}
+
 }  // namespace
 
 namespace mi355 {
 int transpose_gather(const CeedMi355Operator_private *op, void *v, int overwrite, hipStream_t s) {
+  if (op->d_gather_chunk_begin && op->d_sys_active && op->gather_num_chunks > 0) {
+    const unsigned nb = (unsigned)op->gather_num_chunks;
+    if (op->scalar_type == CEED_MI355_FP64)
+      transpose_gather_chunks_kernel<double><<<nb, 256, 0, s>>>(op->d_gather_chunk_begin, op->d_gather_chunk_sys, op->d_sys_active, op->d_trow,
+                                                                (const double *)op->d_evec, (double *)v, overwrite);
+    else
+      transpose_gather_chunks_kernel<float><<<nb, 256, 0, s>>>(op->d_gather_chunk_begin, op->d_gather_chunk_sys, op->d_sys_active, op->d_trow,
+                                                               (const float *)op->d_evec, (float *)v, overwrite);
+    MI355_LAUNCH_CHECK();
+    return CEED_MI355_SUCCESS;
+  }
   const int nb = mi355_blocks(op->l_size, 256, 1 << 20);
   if (op->scalar_type == CEED_MI355_FP64)
     transpose_gather_kernel<double><<<nb, 256, 0, s>>>(op->l_size, op->d_trow, (const double *)op->d_evec, (double *)v, overwrite);
@@ -327,13 +353,23 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
   op->comp_stride = comp_stride;
   op->l_size = l_size;
   op->apply = fn;
-  // E^T mode; by default the transpose map from P = 6 in FP64 and P = 7 in
-  // FP32 (its E-vector and index traffic, (P/p)^3 entries per node, costs
-  // less than the colour launches from there; measured on MI355X), colours below
+  // E^T mode; by default colours only where they are faster than the
+  // transpose map on MI355X: below P = 6 in FP64 (P = 7 in FP32), where the
+  // E-vector round trip of the map costs more than the colour launches, and
+  // from 10M E-vector entries, where each of the (about 8) colour launches
+  // still fills the GPU (at 2M DoFs, p = 4, the map is 1.33x faster; at 8M,
+  // colours are 1.07x faster)
+  const int64_t evec_entries = (int64_t)num_elem * P_1d * P_1d * P_1d * num_comp;
   if (flags & CEED_MI355_PLAN_ATOMIC) op->scatter = CEED_MI355_PLAN_ATOMIC;
   else if (flags & CEED_MI355_PLAN_COLOR) op->scatter = CEED_MI355_PLAN_COLOR;
   else if (flags & CEED_MI355_PLAN_TMAP) op->scatter = 0;
-  else op->scatter = P_1d >= (scalar_type == CEED_MI355_FP32 ? 7 : 6) ? 0 : CEED_MI355_PLAN_COLOR;
+  else op->scatter = P_1d >= (scalar_type == CEED_MI355_FP32 ? 7 : 6) || evec_entries < 10000000 ? 0 : CEED_MI355_PLAN_COLOR;
+  if (!(flags & (CEED_MI355_PLAN_ATOMIC | CEED_MI355_PLAN_COLOR | CEED_MI355_PLAN_TMAP)))
+    if (const char *env = getenv("CEED_MI355_SCATTER")) {  // color | tmap | atomic replaces the default
+      if (!strcmp(env, "color")) op->scatter = CEED_MI355_PLAN_COLOR;
+      else if (!strcmp(env, "tmap")) op->scatter = 0;
+      else if (!strcmp(env, "atomic")) op->scatter = CEED_MI355_PLAN_ATOMIC;
+    }
   // launch: the tuned (layout, elements per workgroup) of this kind of plan if the
   // autotuner has run on one in this process, else the line layout (faster
   // than the column layout at every order measured on MI355X, equal at p <= 2)

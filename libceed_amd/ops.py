@@ -21,8 +21,8 @@ EVAL_NONE, EVAL_INTERP, EVAL_GRAD, EVAL_WEIGHT = 0, 1, 2, 16
 QF_MASS, QF_DIFFUSION, QF_DIFFUSION_KAPPA = 0, 1, 2
 LAYOUT_COLUMNS, LAYOUT_LINES = 0, 1  # thread layouts of the fused kernel
 FP64, FP32 = 0, 1
-# E^T of the fused plan: "auto" (transpose map from P = 6 in FP64 / P = 7 in FP32, colours below), transpose offset map,
-# element colours, hardware atomics
+# E^T of the fused plan: "auto" (colours below P = 6 in FP64 / P = 7 in FP32 on meshes of at least 10M E-vector
+# entries, the transpose map otherwise), transpose offset map, element colours, hardware atomics
 SCATTER_FLAGS = {"auto": 0, "tmap": 8, "color": 2, "atomic": 4}
 GAUSS, GAUSS_LOBATTO = 0, 1
 

@@ -270,8 +270,8 @@ def _user_qf(ceed, kind):
 @pytest.mark.parametrize("kind,p", [(0, 3), (1, 4), (1, 2), (1, 5), (0, 7), (1, 8)])
 def test_user_qfunction(ceed, monkeypatch, fuse, kind, p):
     """A user QFunction (not in the gallery): fused into the run-time compiled
-    line-layout kernel (default; colours below P = 6, the transpose map from
-    there), or the staged E -> B -> D -> B^T -> E^T path with
+    line-layout kernel (default; on meshes this small the E^T is the
+    transpose map), or the staged E -> B -> D -> B^T -> E^T path with
     CEED_MI355_FUSE_QFUNCTIONS=0.  Apply and apply-add."""
     monkeypatch.setenv("CEED_MI355_FUSE_QFUNCTIONS", fuse)
     m = hex_mesh(3 if p < 7 else 2, p, seed=20 + p)
@@ -279,6 +279,28 @@ def test_user_qfunction(ceed, monkeypatch, fuse, kind, p):
     qdata, qd_rstr = build_qdata(ceed, m, Q, kind)
     qref = oracle.hex_build_qdata(kind, m, Q)
     rng = np.random.default_rng(p)
+    u = rng.uniform(-1, 1, m.num_nodes)
+    v, _ = apply_operator(ceed, m, P, Q, kind, 1, _user_qf(ceed, kind), qdata, qd_rstr, u)
+    ref = oracle.hex_apply(kind, 1, m.num_elem, P, Q, m.offsets, m.num_nodes, qref, u)
+    assert rel_l2(v, ref) < RTOL
+    v0 = rng.uniform(-1, 1, u.size)
+    v2, _ = apply_operator(ceed, m, P, Q, kind, 1, _user_qf(ceed, kind), qdata, qd_rstr, u, v0=v0)
+    assert rel_l2(v2, ref + v0) < RTOL
+
+
+@pytest.mark.parametrize("scatter", ["color", "tmap", "atomic"])
+@pytest.mark.parametrize("kind,p", [(0, 3), (1, 4), (1, 6)])
+def test_user_qfunction_scatter_modes(ceed, monkeypatch, scatter, kind, p):
+    """The run-time fused kernel in each E^T mode (CEED_MI355_SCATTER replaces
+    the size-based default): colour launches with first-touch overwrite, the
+    transpose map, atomics.  Apply and apply-add against the oracle."""
+    monkeypatch.setenv("CEED_MI355_FUSE_QFUNCTIONS", "1")
+    monkeypatch.setenv("CEED_MI355_SCATTER", scatter)
+    m = hex_mesh(4, p, seed=40 + p)
+    P, Q = p + 1, p + 2
+    qdata, qd_rstr = build_qdata(ceed, m, Q, kind)
+    qref = oracle.hex_build_qdata(kind, m, Q)
+    rng = np.random.default_rng(p + 7)
     u = rng.uniform(-1, 1, m.num_nodes)
     v, _ = apply_operator(ceed, m, P, Q, kind, 1, _user_qf(ceed, kind), qdata, qd_rstr, u)
     ref = oracle.hex_apply(kind, 1, m.num_elem, P, Q, m.offsets, m.num_nodes, qref, u)
