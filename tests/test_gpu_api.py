@@ -219,9 +219,14 @@ def test_qdata_setup_operator(ceed, kind):
 GALLERY = {(0, 1): "MassApply", (0, 3): "Vector3MassApply", (1, 1): "Poisson3DApply", (1, 3): "Vector3Poisson3DApply"}
 
 
+@pytest.mark.parametrize("scatter", ["", "color", "atomic"])
 @pytest.mark.parametrize("kind,nc,p", [(0, 1, 3), (1, 1, 4), (0, 3, 2), (1, 3, 3), (1, 1, 7)])
-def test_gallery_operator_fused(ceed, kind, nc, p):
-    """Gallery apply QFunctions take the fused single-kernel path."""
+def test_gallery_operator_fused(ceed, monkeypatch, scatter, kind, nc, p):
+    """Gallery apply QFunctions take the fused single-kernel path, in the
+    default E^T mode (on meshes this small: the transpose map) and with
+    CEED_MI355_SCATTER forcing colours or atomics."""
+    if scatter:
+        monkeypatch.setenv("CEED_MI355_SCATTER", scatter)
     m = hex_mesh(3, p, seed=p)
     P, Q = p + 1, p + 2
     qdata, qd_rstr = build_qdata(ceed, m, Q, kind)
