@@ -50,7 +50,7 @@ enum { CEED_MI355_QF_MASS = 0, CEED_MI355_QF_DIFFUSION = 1, CEED_MI355_QF_DIFFUS
 enum { CEED_MI355_FIELD_INTERP = 0, CEED_MI355_FIELD_GRAD = 1, CEED_MI355_FIELD_PASSIVE = 2 };
 
 /* E^T mode flags of CeedMi355OperatorCreateHex (0: transpose map for P >= 6, colours below). */
-enum { CEED_MI355_PLAN_COLOR = 2, CEED_MI355_PLAN_ATOMIC = 4, CEED_MI355_PLAN_TMAP = 8 };
+enum { CEED_MI355_PLAN_COLOR = 2, CEED_MI355_PLAN_ATOMIC = 4, CEED_MI355_PLAN_TMAP = 8, CEED_MI355_PLAN_SPLIT = 16 };
 
 /* Geometric-factor builders (gallery Mass3DBuild / Poisson3DBuild). */
 enum { CEED_MI355_BUILD_MASS = 0, CEED_MI355_BUILD_DIFFUSION = 1 };
@@ -111,7 +111,11 @@ int CeedMi355BasisDestroy(CeedMi355Basis *basis);
 typedef struct CeedMi355Operator_private *CeedMi355Operator;
 
 /* Plan for a 3D tensor hex operator with a gallery pointwise D, applied by
- * one fused kernel launch.  E^T (flags; 0 picks TMAP for P_1d >= 6, COLOR below):
+ * one fused kernel launch.  E^T (flags; 0 picks the mode measured fastest on
+ * MI355X for the order, precision and size: COLOR for P_1d <= 5 (FP64) or
+ * <= 6 (FP32) from 10M E-vector entries, SPLIT for P_1d = 7, 8 in FP64 from
+ * 10M entries, TMAP otherwise; CEED_MI355_SCATTER=color|tmap|split|atomic
+ * replaces the choice):
  *   CEED_MI355_PLAN_TMAP   the fused kernel stores every contribution into
  *                          an E-vector sorted by L entry (slots from the
  *                          transpose offset map); a gather kernel sums each
@@ -122,6 +126,13 @@ typedef struct CeedMi355Operator_private *CeedMi355Operator;
  *   CEED_MI355_PLAN_COLOR  elements coloured so that no two of a colour share
  *                          an L entry; one launch per colour adds in place
  *                          (no atomics, bitwise reproducible);
+ *   CEED_MI355_PLAN_SPLIT  entries reached by at most two contributions are
+ *                          summed by atomic adds onto a zeroed target (a
+ *                          two-term sum does not depend on the order), the
+ *                          others by the transpose map restricted to them:
+ *                          every entry summed in the reference's order, Apply
+ *                          bitwise equal to TMAP; ApplyAdd forms A u in a
+ *                          scratch vector, then adds it;
  *   CEED_MI355_PLAN_ATOMIC hardware atomic adds, sums in arrival order.
  * Supports Q_1d in {P_1d, P_1d + 1}, P_1d in 2..9.  scalar_type selects FP64
  * or FP32 storage and arithmetic for u, v and q_data.  Replaces the
@@ -200,6 +211,10 @@ int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int32_t *best_
 /* Number of element colours, and the element count of each (host array of at
  * least num_colors entries, may be NULL). */
 int CeedMi355OperatorGetColors(CeedMi355Operator op, int32_t *num_colors, int32_t *color_sizes_host);
+/* The plan's E^T mode (CEED_MI355_PLAN_TMAP, _COLOR, _SPLIT or _ATOMIC), as
+ * chosen at creation.  No counterpart in the reference (its backends have one
+ * fixed E^T each). */
+int CeedMi355OperatorGetScatter(CeedMi355Operator op, int32_t *mode);
 int CeedMi355OperatorDestroy(CeedMi355Operator *op);
 
 /* ------------------------------------------------------------------------ */

@@ -43,6 +43,7 @@ SIGNATURES = {
     "CeedMi355OperatorAssembleDiagonal": (c_int, [vp, vp, c_int, vp]),
     "CeedMi355OperatorApply": (c_int, [vp, vp, vp, vp]),
     "CeedMi355OperatorGetColors": (c_int, [vp, C.POINTER(c_i32), vp]),
+    "CeedMi355OperatorGetScatter": (c_int, [vp, C.POINTER(c_i32)]),
     "CeedMi355OperatorDestroy": (c_int, [C.POINTER(vp)]),
     "CeedGaussQuadrature": (c_int, [c_i32, vp, vp]),
     "CeedLobattoQuadrature": (c_int, [c_i32, vp, vp]),

@@ -20,8 +20,8 @@ template <int P, int Q>
 __global__ void __launch_bounds__(256) hex_adjoint_kernel(int64_t elem_begin, int64_t elem_end, int scatter, const int32_t *__restrict__ offsets,
                                                           const int32_t *__restrict__ perm, const double *__restrict__ q_data,
                                                           const double *__restrict__ mats, const double *__restrict__ u,
-                                                          const double *__restrict__ lam, double *__restrict__ v, const int32_t *__restrict__ tpos,
-                                                          double *__restrict__ g) {
+                                                          const double *__restrict__ lam, double *__restrict__ v, double *__restrict__ evec,
+                                                          const int32_t *__restrict__ tpos, double *__restrict__ g) {
   using S = HexShape<P, Q>;
   constexpr int T1 = S::T1, NT = S::NT, SLAB = S::SLAB, P3 = S::P3, Q3 = S::Q3, QC = 7;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -176,11 +176,15 @@ __global__ void __launch_bounds__(256) hex_adjoint_kernel(int64_t elem_begin, in
       double s = 0;
 #pragma unroll
       for (int a = 0; a < Q; a++) s += sB[a * P + tx] * S1[sidx(z, ty, a)];
-      if (scatter == kScatterEvec) {  // L-entry-sorted E-vector
-        v[tpos[e * (int64_t)P3 + (z * P + ty) * P + tx]] = s;
+      if (scatter == kScatterEvec || (scatter == kScatterSplit && (off[z] & 1))) {  // L-entry-sorted E-vector
+        evec[tpos[e * (int64_t)P3 + (z * P + ty) * P + tx]] = s;
         continue;
       }
       double *dst = &v[off[z] >> 1];
+      if (scatter == kScatterSplit) {  // at most two contributions: onto the zeroed v
+        unsafeAtomicAdd(dst, s);
+        continue;
+      }
       if (scatter == kScatterAtomic) unsafeAtomicAdd(dst, s);
       else if (scatter == kScatterOverwrite && (off[z] & 1)) *dst = s;
       else *dst += s;
@@ -197,7 +201,7 @@ template <int P, int Q>
 __global__ void __launch_bounds__(256) hex_adjoint_lines_kernel(int64_t elem_begin, int64_t elem_end, int scatter, const int32_t *__restrict__ offsets,
                                                                 const int32_t *__restrict__ perm, const double *__restrict__ q_data,
                                                                 const double *__restrict__ mats, const double *__restrict__ u,
-                                                                const double *__restrict__ lam, double *__restrict__ v,
+                                                                const double *__restrict__ lam, double *__restrict__ v, double *__restrict__ evec,
                                                                 const int32_t *__restrict__ tpos, double *__restrict__ g) {
   constexpr int  NT = Q * Q, SLAB = Q * Q * Q, P3 = P * P * P, Q3 = Q * Q * Q, QC = 7, NS = 5;
   constexpr bool kRowWise = Q >= 7;
@@ -419,7 +423,13 @@ __global__ void __launch_bounds__(256) hex_adjoint_lines_kernel(int64_t elem_beg
     for (int z = 0; z < P; z++) s[z] = S1[at(z, cy, cx)];
     if (scatter == kScatterEvec) {
 #pragma unroll
-      for (int z = 0; z < P; z++) v[tpos[e * (int64_t)P3 + (z * P + cy) * P + cx]] = s[z];
+      for (int z = 0; z < P; z++) evec[tpos[e * (int64_t)P3 + (z * P + cy) * P + cx]] = s[z];
+    } else if (scatter == kScatterSplit) {  // shared entries to their E-vector slot, the rest onto the zeroed v
+#pragma unroll
+      for (int z = 0; z < P; z++) {
+        if (off[z] & 1) evec[tpos[e * (int64_t)P3 + (z * P + cy) * P + cx]] = s[z];
+        else unsafeAtomicAdd(&v[off[z] >> 1], s[z]);
+      }
     } else if (scatter == kScatterAtomic) {
 #pragma unroll
       for (int z = 0; z < P; z++) unsafeAtomicAdd(&v[off[z] >> 1], s[z]);
@@ -449,17 +459,19 @@ int launch_adjoint(const CeedMi355Operator_private *op, const double *u, const d
     const int64_t b = op->color_start[c], e = op->color_start[c + 1];
     if (e <= b) continue;
     const int64_t grid    = (e - b + epb - 1) / epb;
-    const int     scatter = tmap ? kScatterEvec : (op->scatter == CEED_MI355_PLAN_ATOMIC ? kScatterAtomic : kScatterOverwrite);
+    const int     scatter = tmap ? kScatterEvec
+                                 : op->scatter == CEED_MI355_PLAN_SPLIT ? kScatterSplit
+                                                                        : (op->scatter == CEED_MI355_PLAN_ATOMIC ? kScatterAtomic : kScatterOverwrite);
     if (lines)
       hex_adjoint_lines_kernel<P, Q><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, scatter, op->d_offsets, op->d_perm, (const double *)op->d_qdata,
-                                                                          (const double *)op->d_mats, u, lam, tmap ? (double *)op->d_evec : v,
-                                                                          op->d_tpos, g);
+                                                                          (const double *)op->d_mats, u, lam, v, (double *)op->d_evec, op->d_tpos, g);
     else
       hex_adjoint_kernel<P, Q><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, scatter, op->d_offsets, op->d_perm, (const double *)op->d_qdata,
-                                                                    (const double *)op->d_mats, u, lam, tmap ? (double *)op->d_evec : v, op->d_tpos, g);
+                                                                    (const double *)op->d_mats, u, lam, v, (double *)op->d_evec, op->d_tpos, g);
   }
   MI355_LAUNCH_CHECK();
   if (tmap) return transpose_gather(op, v, 1, s);
+  if (op->scatter == CEED_MI355_PLAN_SPLIT) return split_finish(op, v, 1, s);
   return CEED_MI355_SUCCESS;
 }
 

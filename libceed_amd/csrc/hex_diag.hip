@@ -18,7 +18,8 @@ namespace {
 template <int P, int Q, typename T>
 __global__ void __launch_bounds__(128) hex_diag_kernel(int64_t elem_begin, int64_t elem_end, int scatter, int qf, int num_comp,
                                                        const int32_t *__restrict__ offsets, int32_t comp_stride, const T *__restrict__ q_data,
-                                                       int q_comp, const T *__restrict__ mats, T *__restrict__ v, const int32_t *__restrict__ tpos) {
+                                                       int q_comp, const T *__restrict__ mats, T *__restrict__ v, T *__restrict__ evec,
+                                                       const int32_t *__restrict__ tpos) {
   constexpr int P3 = P * P * P, Q3 = Q * Q * Q;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *sB  = reinterpret_cast<T *>(lds_raw);  // Q x P
@@ -61,11 +62,15 @@ __global__ void __launch_bounds__(128) hex_diag_kernel(int64_t elem_begin, int64
     }
     const int32_t o = offsets[e * P3 + i];
     for (int k = 0; k < num_comp; k++) {
-      if (scatter == kScatterEvec) {  // L-entry-sorted E-vector
-        v[tpos[(e * num_comp + k) * (int64_t)P3 + i]] = acc;
+      if (scatter == kScatterEvec || (scatter == kScatterSplit && (o & 1))) {  // L-entry-sorted E-vector
+        evec[tpos[(e * num_comp + k) * (int64_t)P3 + i]] = acc;
         continue;
       }
       T *dst = &v[(o >> 1) + (int64_t)k * comp_stride];
+      if (scatter == kScatterSplit) {  // at most two contributions: onto the zeroed target
+        unsafeAtomicAdd(dst, acc);
+        continue;
+      }
       if (scatter == kScatterAtomic) unsafeAtomicAdd(dst, acc);
       else if (scatter == kScatterOverwrite && (o & 1)) *dst = acc;
       else *dst += acc;
@@ -82,21 +87,26 @@ template <int P, int Q, typename T>
 int launch_diag(const CeedMi355Operator_private *op, void *v, int add, hipStream_t s) {
   const size_t lds  = sizeof(T) * (size_t)(2 * Q * P + op->q_comp * Q * Q * Q);
   const bool   tmap = op->scatter == 0;
-  if (!add && !tmap && !op->covers_all) {
-    diag_zero<T><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, (T *)v);
+  const bool   split = op->scatter == CEED_MI355_PLAN_SPLIT;
+  // the split mode's atomic adds land on a zeroed target: v, or with add the scratch vector
+  T *target = split && add ? (T *)op->d_scratch : (T *)v;
+  if ((!add && !tmap && !op->covers_all) || (split && add)) {
+    diag_zero<T><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, target);
     MI355_LAUNCH_CHECK();
   }
   for (size_t c = 0; c + 1 < op->color_start.size(); c++) {
     const int64_t b = op->color_start[c], e = op->color_start[c + 1];
     if (e <= b) continue;
     const int scatter = tmap ? kScatterEvec
-                             : (op->scatter == CEED_MI355_PLAN_ATOMIC ? kScatterAtomic : (add ? kScatterAdd : kScatterOverwrite));
+                        : split ? kScatterSplit
+                                : (op->scatter == CEED_MI355_PLAN_ATOMIC ? kScatterAtomic : (add ? kScatterAdd : kScatterOverwrite));
     hex_diag_kernel<P, Q, T><<<(unsigned)(e - b), 128, lds, s>>>(b, e, scatter, op->qf, op->num_comp, op->d_offsets, op->comp_stride,
-                                                               (const T *)op->d_qdata, op->q_comp, (const T *)op->d_mats,
-                                                               tmap ? (T *)op->d_evec : (T *)v, op->d_tpos);
+                                                               (const T *)op->d_qdata, op->q_comp, (const T *)op->d_mats, target,
+                                                               (T *)op->d_evec, op->d_tpos);
   }
   MI355_LAUNCH_CHECK();
   if (tmap) return transpose_gather(op, v, !add, s);
+  if (split) return split_finish(op, v, !add, s);
   return CEED_MI355_SUCCESS;
 }
 

@@ -27,7 +27,7 @@ enum { kQfMass = 0, kQfDiffusion = 1, kQfDiffusionKappa = 2, kQfJit = 3 };
 // first-touch overwrite (Apply), hardware atomics (one launch, any order), or
 // store of each contribution into an E-vector sorted by L entry (one launch;
 // a separate gather sums each L entry's contiguous run of contributions).
-enum { kScatterAdd = 0, kScatterOverwrite = 1, kScatterAtomic = 2, kScatterEvec = 3 };
+enum { kScatterAdd = 0, kScatterOverwrite = 1, kScatterAtomic = 2, kScatterEvec = 3, kScatterSplit = 4 };
 
 template <int QF>
 struct QfTraits;
@@ -93,7 +93,8 @@ template <int P, int Q, int NCOMP, int QF, typename T>
 __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int64_t elem_end, int scatter,
                                                          const int32_t *__restrict__ offsets, int32_t comp_stride,
                                                          const T *__restrict__ q_data, const T *__restrict__ mats, const T *__restrict__ u,
-                                                         T *__restrict__ v, const int32_t *__restrict__ tpos, const int32_t *__restrict__ elem_sys,
+                                                         T *__restrict__ v, T *__restrict__ evec, const int32_t *__restrict__ tpos,
+                                                         const int32_t *__restrict__ elem_sys,
                                                          const int32_t *__restrict__ sys_active) {
   using S = HexShape<P, Q>;
   constexpr int T1 = S::T1, NT = S::NT, SLAB = S::SLAB, P3 = S::P3, Q3 = S::Q3, QC = QfTraits<QF>::kComps;
@@ -295,10 +296,16 @@ __global__ void __launch_bounds__(256) hex_apply_kernel(int64_t elem_begin, int6
 #pragma unroll
         for (int z = 0; z < P; z++) pos[z] = tpos[(e * NCOMP + comp) * (int64_t)P3 + (z * P + ty) * P + tx];
 #pragma unroll
-        for (int z = 0; z < P; z++) v[pos[z]] = s[z];
+        for (int z = 0; z < P; z++) evec[pos[z]] = s[z];
       } else if (scatter == kScatterAtomic) {  // all elements in one launch
 #pragma unroll
         for (int z = 0; z < P; z++) unsafeAtomicAdd(&v[(off[z] >> 1) + coff], s[z]);
+      } else if (scatter == kScatterSplit) {  // shared entries to their E-vector slot, the rest onto the zeroed target
+#pragma unroll
+        for (int z = 0; z < P; z++) {
+          if (off[z] & 1) evec[tpos[(e * NCOMP + comp) * (int64_t)P3 + (z * P + ty) * P + tx]] = s[z];
+          else unsafeAtomicAdd(&v[(off[z] >> 1) + coff], s[z]);
+        }
       } else {
         // all reads of v before the writes, so the loads are in flight together;
         // Apply's first element to reach an entry (first-touch bit) writes it without reading

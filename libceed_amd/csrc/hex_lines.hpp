@@ -62,7 +62,8 @@ template <int P, int Q, int NCOMP, int QF, typename T>
 __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin, int64_t elem_end, int scatter,
                                                                const int32_t *__restrict__ offsets, int32_t comp_stride,
                                                                const T *__restrict__ q_data, const T *__restrict__ mats,
-                                                               const T *__restrict__ u, T *__restrict__ v, const int32_t *__restrict__ tpos,
+                                                               const T *__restrict__ u, T *__restrict__ v, T *__restrict__ evec,
+                                                               const int32_t *__restrict__ tpos,
                                                                const int32_t *__restrict__ elem_sys, const int32_t *__restrict__ sys_active) {
   constexpr int NT = Q * Q, SLAB = Q * Q * Q, P3 = P * P * P, Q3 = Q * Q * Q, QC = QfTraits<QF>::kComps;
   constexpr int NS = LineSlabs<QF>::kSlabs;
@@ -333,10 +334,16 @@ __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin
 #pragma unroll
         for (int z = 0; z < P; z++) pos[z] = tpos[(e * NCOMP + comp) * (int64_t)P3 + (z * P + cy) * P + cx];
 #pragma unroll
-        for (int z = 0; z < P; z++) v[pos[z]] = s[z];
+        for (int z = 0; z < P; z++) evec[pos[z]] = s[z];
       } else if (scatter == kScatterAtomic) {
 #pragma unroll
         for (int z = 0; z < P; z++) unsafeAtomicAdd(&v[(off[z] >> 1) + coff], s[z]);
+      } else if (scatter == kScatterSplit) {  // shared entries to their E-vector slot, the rest onto the zeroed target
+#pragma unroll
+        for (int z = 0; z < P; z++) {
+          if (off[z] & 1) evec[tpos[(e * NCOMP + comp) * (int64_t)P3 + (z * P + cy) * P + cx]] = s[z];
+          else unsafeAtomicAdd(&v[(off[z] >> 1) + coff], s[z]);
+        }
       } else {
         const bool overwrite = scatter == kScatterOverwrite;
         T          old[P];

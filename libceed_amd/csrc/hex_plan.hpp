@@ -17,6 +17,10 @@ namespace mi355 {
 using ApplyFn = int (*)(const CeedMi355Operator_private *op, const void *u, void *v, int overwrite, hipStream_t s);
 // v (=|+=) E^T evec by the plan's transpose map (operator_hex.hip)
 int transpose_gather(const CeedMi355Operator_private *op, void *v, int overwrite, hipStream_t s);
+// The split mode's second step: Apply (overwrite) writes the shared entries of
+// v; ApplyAdd adds the scratch target and the shared sums into v.
+int split_finish(const CeedMi355Operator_private *op, void *v, int overwrite, hipStream_t s);
+int fill_zero(const CeedMi355Operator_private *op, void *x, hipStream_t s);
 
 // Kernel launcher for one (P, Q, num_comp, QF, scalar type); nullptr when no instantiation exists.
 ApplyFn pick_apply_f64(int P, int Q, int num_comp, int qf);
@@ -64,6 +68,16 @@ struct CeedMi355Operator_private {
   // ascending (elem, comp, node) order
   int32_t *d_trow = nullptr, *d_tpos = nullptr;
   void    *d_evec = nullptr;  // E-vector sorted by L entry, plan scalar type
+  // multiplicity split (scatter == CEED_MI355_PLAN_SPLIT): L entries reached by
+  // at most two contributions are summed by atomic adds onto a zeroed target
+  // (a two-term sum does not depend on the order); the others ("shared",
+  // offset bit 0 set) take the transpose map above, restricted to them
+  // (d_tpos holds a slot for shared contributions only, d_trow runs for
+  // shared entries only); d_srows lists the shared L entries; d_scratch is
+  // the zeroed target of ApplyAdd, added to v afterwards
+  int32_t *d_srows    = nullptr;
+  int64_t  num_shared = 0;
+  void    *d_scratch  = nullptr;
   // set by a batched solver: system of each (packed) element, and the
   // solver's per-system active flags; elements of inactive systems are skipped
   const int32_t *d_elem_sys = nullptr, *d_sys_active = nullptr;
@@ -93,23 +107,33 @@ int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v
   const bool   lines = op->layout == kLayoutLines;
   const size_t lds   = sizeof(T) * (size_t)(lines ? hex_lines_lds_entries<P, Q, QF>(epb) : hex_lds_entries<P, Q, QF>(epb));
   const bool   tmap  = op->scatter == 0;
+  const bool   split = op->scatter == CEED_MI355_PLAN_SPLIT;
+  // the split mode's atomic adds land on a zeroed target: v in Apply (zeroed
+  // by CeedMi355OperatorApply), the scratch vector in ApplyAdd
+  T *target = split && !overwrite ? (T *)op->d_scratch : (T *)v;
+  if (split && !overwrite) {
+    const int err = fill_zero(op, target, s);
+    if (err) return err;
+  }
   for (size_t c = 0; c + 1 < op->color_start.size(); c++) {
     const int64_t b = op->color_start[c], e = op->color_start[c + 1];
     if (e <= b) continue;
     const int64_t grid    = (e - b + epb - 1) / epb;
     const int     scatter = tmap ? kScatterEvec
-                                 : (op->scatter == CEED_MI355_PLAN_ATOMIC ? kScatterAtomic : (overwrite ? kScatterOverwrite : kScatterAdd));
+                                 : split ? kScatterSplit
+                                         : (op->scatter == CEED_MI355_PLAN_ATOMIC ? kScatterAtomic : (overwrite ? kScatterOverwrite : kScatterAdd));
     if (lines)
       hex_apply_lines_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(
-          b, e, scatter, op->d_offsets, op->comp_stride, (const T *)op->d_qdata, (const T *)op->d_mats, (const T *)u,
-          tmap ? (T *)op->d_evec : (T *)v, op->d_tpos, op->d_elem_sys, op->d_sys_active);
+          b, e, scatter, op->d_offsets, op->comp_stride, (const T *)op->d_qdata, (const T *)op->d_mats, (const T *)u, target, (T *)op->d_evec,
+          op->d_tpos, op->d_elem_sys, op->d_sys_active);
     else
       hex_apply_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, scatter, op->d_offsets, op->comp_stride, (const T *)op->d_qdata,
-                                                                            (const T *)op->d_mats, (const T *)u, tmap ? (T *)op->d_evec : (T *)v,
+                                                                            (const T *)op->d_mats, (const T *)u, target, (T *)op->d_evec,
                                                                             op->d_tpos, op->d_elem_sys, op->d_sys_active);
   }
   MI355_LAUNCH_CHECK();
   if (tmap) return transpose_gather(op, v, overwrite, s);
+  if (split) return split_finish(op, v, overwrite, s);
   return CEED_MI355_SUCCESS;
 }
 

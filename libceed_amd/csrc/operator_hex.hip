@@ -273,6 +273,70 @@ int build_transpose_map(CeedMi355Operator op, const int32_t *offsets) {
   return CEED_MI355_SUCCESS;
 }
 
+// The split mode's maps (element order = caller order, packed offsets with
+// the shared bit): slots in the shared E-vector for shared contributions
+// only, sorted by L entry and, within one entry, ascending (elem, comp, node);
+// row runs for shared entries; the list of shared entries; ApplyAdd's target.
+int build_split_map(CeedMi355Operator op, const int32_t *packed) {
+  const int64_t es = (int64_t)op->P * op->P * op->P, nnz = (int64_t)op->num_elem * op->num_comp * es;
+  MI355_CHECK(nnz < INT32_MAX && op->l_size < INT32_MAX, CEED_MI355_ERROR_UNSUPPORTED,
+              "the split E^T needs fewer than 2^31 E-vector entries (have %lld); use CEED_MI355_PLAN_COLOR", (long long)nnz);
+  std::vector<int32_t> row((size_t)op->l_size + 1, 0), pos((size_t)nnz + 1, -1), srows;
+  for (int64_t e = 0; e < op->num_elem; e++)
+    for (int64_t k = 0; k < op->num_comp; k++)
+      for (int64_t i = 0; i < es; i++) {
+        const int32_t o = packed[e * es + i];
+        if (o & 1) row[(size_t)((o >> 1) + k * op->comp_stride) + 1]++;
+      }
+  for (int64_t t = 0; t < op->l_size; t++) {
+    if (row[t + 1] > 0) srows.push_back((int32_t)t);
+    row[t + 1] += row[t];
+  }
+  std::vector<int32_t> cursor(row.begin(), row.end() - 1);
+  for (int64_t e = 0; e < op->num_elem; e++)
+    for (int64_t k = 0; k < op->num_comp; k++)
+      for (int64_t i = 0; i < es; i++) {
+        const int32_t o = packed[e * es + i];
+        if (o & 1) pos[(size_t)((e * op->num_comp + k) * es + i)] = cursor[(o >> 1) + k * op->comp_stride]++;
+      }
+  const int64_t nshared = row[(size_t)op->l_size];
+  op->num_shared        = (int64_t)srows.size();
+  MI355_HIP(hipMalloc(&op->d_trow, sizeof(int32_t) * row.size()));
+  MI355_HIP(hipMalloc(&op->d_tpos, sizeof(int32_t) * pos.size()));
+  MI355_HIP(hipMalloc(&op->d_srows, sizeof(int32_t) * (srows.size() + 1)));
+  MI355_HIP(hipMalloc(&op->d_evec, (size_t)op->scalar_bytes * (nshared + 1)));
+  MI355_HIP(hipMalloc(&op->d_scratch, (size_t)op->scalar_bytes * (op->l_size + 1)));
+  MI355_HIP(hipMemcpy(op->d_trow, row.data(), sizeof(int32_t) * row.size(), hipMemcpyHostToDevice));
+  MI355_HIP(hipMemcpy(op->d_tpos, pos.data(), sizeof(int32_t) * pos.size(), hipMemcpyHostToDevice));
+  if (!srows.empty()) MI355_HIP(hipMemcpy(op->d_srows, srows.data(), sizeof(int32_t) * srows.size(), hipMemcpyHostToDevice));
+  return CEED_MI355_SUCCESS;
+}
+
+// split mode, Apply: each shared entry = the ordered sum of its run
+template <typename T>
+__global__ void __launch_bounds__(256) split_gather_kernel(int64_t num_shared, const int32_t *__restrict__ srows, const int32_t *__restrict__ row,
+                                                           const T *__restrict__ evec, T *__restrict__ v) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < num_shared; j += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t t   = srows[j];
+    T             acc = 0;
+    for (int32_t m = row[t]; m < row[t + 1]; m++) acc += evec[m];
+    v[t] = acc;
+  }
+}
+
+// split mode, ApplyAdd: v += (the ordered run of a shared entry, else the atomic target w)
+template <typename T>
+__global__ void __launch_bounds__(256) split_add_kernel(int64_t l_size, const int32_t *__restrict__ row, const T *__restrict__ evec,
+                                                        const T *__restrict__ w, T *__restrict__ v) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < l_size; t += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t b = row[t], e = row[t + 1];
+    T             a = 0;
+    if (b == e) a = w[t];
+    for (int32_t m = b; m < e; m++) a += evec[m];
+    v[t] += a;
+  }
+}
+
 // v[t] (= or +=) the sum of t's contiguous run of the sorted E-vector, in order
 template <typename T>
 __global__ void __launch_bounds__(256) transpose_gather_kernel(int64_t l_size, const int32_t *__restrict__ row, const T *__restrict__ evec,
@@ -302,6 +366,31 @@ __global__ void __launch_bounds__(256) transpose_gather_chunks_kernel(const int6
 }  // namespace
 
 namespace mi355 {
+int fill_zero(const CeedMi355Operator_private *op, void *x, hipStream_t s) {
+  if (op->scalar_type == CEED_MI355_FP64) fill_kernel<double><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, 0.0, (double *)x);
+  else fill_kernel<float><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, 0.0f, (float *)x);
+  MI355_LAUNCH_CHECK();
+  return CEED_MI355_SUCCESS;
+}
+
+int split_finish(const CeedMi355Operator_private *op, void *v, int overwrite, hipStream_t s) {
+  if (overwrite) {
+    if (op->num_shared == 0) return CEED_MI355_SUCCESS;
+    const int nb = mi355_blocks(op->num_shared, 256, 1 << 20);
+    if (op->scalar_type == CEED_MI355_FP64)
+      split_gather_kernel<double><<<nb, 256, 0, s>>>(op->num_shared, op->d_srows, op->d_trow, (const double *)op->d_evec, (double *)v);
+    else split_gather_kernel<float><<<nb, 256, 0, s>>>(op->num_shared, op->d_srows, op->d_trow, (const float *)op->d_evec, (float *)v);
+  } else {
+    const int nb = mi355_blocks(op->l_size, 256, 1 << 20);
+    if (op->scalar_type == CEED_MI355_FP64)
+      split_add_kernel<double><<<nb, 256, 0, s>>>(op->l_size, op->d_trow, (const double *)op->d_evec, (const double *)op->d_scratch, (double *)v);
+    else
+      split_add_kernel<float><<<nb, 256, 0, s>>>(op->l_size, op->d_trow, (const float *)op->d_evec, (const float *)op->d_scratch, (float *)v);
+  }
+  MI355_LAUNCH_CHECK();
+  return CEED_MI355_SUCCESS;
+}
+
 int transpose_gather(const CeedMi355Operator_private *op, void *v, int overwrite, hipStream_t s) {
   if (op->d_gather_chunk_begin && op->d_sys_active && op->gather_num_chunks > 0) {
     const unsigned nb = (unsigned)op->gather_num_chunks;
@@ -358,16 +447,22 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
   // E-vector round trip of the map costs more than the colour launches, and
   // from 10M E-vector entries, where each of the (about 8) colour launches
   // still fills the GPU (at 2M DoFs, p = 4, the map is 1.33x faster; at 8M,
-  // colours are 1.07x faster)
+  // colours are 1.07x faster).  From 10M entries at P = 7 and 8 in FP64 the
+  // multiplicity split (same Apply bits as the map) is faster still: 1.08x
+  // at p = 6, 1.03x at p = 7, 8M DoFs.
   const int64_t evec_entries = (int64_t)num_elem * P_1d * P_1d * P_1d * num_comp;
-  if (flags & CEED_MI355_PLAN_ATOMIC) op->scatter = CEED_MI355_PLAN_ATOMIC;
+  const bool    big          = evec_entries >= 10000000;
+  if (flags & CEED_MI355_PLAN_SPLIT) op->scatter = CEED_MI355_PLAN_SPLIT;
+  else if (flags & CEED_MI355_PLAN_ATOMIC) op->scatter = CEED_MI355_PLAN_ATOMIC;
   else if (flags & CEED_MI355_PLAN_COLOR) op->scatter = CEED_MI355_PLAN_COLOR;
   else if (flags & CEED_MI355_PLAN_TMAP) op->scatter = 0;
-  else op->scatter = P_1d >= (scalar_type == CEED_MI355_FP32 ? 7 : 6) || evec_entries < 10000000 ? 0 : CEED_MI355_PLAN_COLOR;
-  if (!(flags & (CEED_MI355_PLAN_ATOMIC | CEED_MI355_PLAN_COLOR | CEED_MI355_PLAN_TMAP)))
-    if (const char *env = getenv("CEED_MI355_SCATTER")) {  // color | tmap | atomic replaces the default
+  else if (big && scalar_type == CEED_MI355_FP64 && (P_1d == 7 || P_1d == 8)) op->scatter = CEED_MI355_PLAN_SPLIT;
+  else op->scatter = P_1d >= (scalar_type == CEED_MI355_FP32 ? 7 : 6) || !big ? 0 : CEED_MI355_PLAN_COLOR;
+  if (!(flags & (CEED_MI355_PLAN_ATOMIC | CEED_MI355_PLAN_COLOR | CEED_MI355_PLAN_TMAP | CEED_MI355_PLAN_SPLIT)))
+    if (const char *env = getenv("CEED_MI355_SCATTER")) {  // color | tmap | split | atomic replaces the default
       if (!strcmp(env, "color")) op->scatter = CEED_MI355_PLAN_COLOR;
       else if (!strcmp(env, "tmap")) op->scatter = 0;
+      else if (!strcmp(env, "split")) op->scatter = CEED_MI355_PLAN_SPLIT;
       else if (!strcmp(env, "atomic")) op->scatter = CEED_MI355_PLAN_ATOMIC;
     }
   // launch: the tuned (layout, elements per workgroup) of this kind of plan if the
@@ -414,9 +509,11 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
     std::vector<int64_t> cursor(op->color_start.begin(), op->color_start.end() - 1);
     for (int64_t e = 0; e < num_elem; e++) op->perm[cursor[color[e]]++] = (int32_t)e;
   }
-  // offsets in colour order, encoded 2 * node + first_touch: the first element (in launch order)
-  // reaching a node may overwrite v in CeedOperatorApply instead of accumulating
+  // offsets in colour order, encoded 2 * node + bit: with colours the bit marks the first
+  // element (in launch order) reaching a node, which may overwrite v in CeedOperatorApply
+  // instead of accumulating; in the split mode it marks nodes of more than two contributions
   std::vector<int32_t> packed((size_t)num_elem * es);
+  const bool           split = op->scatter == CEED_MI355_PLAN_SPLIT;
   {
     int64_t n_nodes = 0;
     for (int64_t i = 0; i < (int64_t)num_elem * es; i++) n_nodes = std::max<int64_t>(n_nodes, (int64_t)offsets_host[i] + 1);
@@ -424,12 +521,14 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
       delete op;
       MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED, "more than 2^30 nodes");
     }
-    std::vector<char> touched(n_nodes, 0);
+    std::vector<char>    touched(n_nodes, 0);
+    std::vector<int32_t> mult(split ? n_nodes : 0, 0);
+    if (split)
+      for (int64_t i = 0; i < (int64_t)num_elem * es; i++) mult[offsets_host[i]]++;
     for (int64_t p = 0; p < num_elem; p++)
       for (int i = 0; i < es; i++) {
         const int32_t node = offsets_host[(int64_t)op->perm[p] * es + i];
-        // bit 0 (colours): first element, in launch order, to reach the node
-        packed[p * es + i] = 2 * node + (colored && !touched[node] ? 1 : 0);
+        packed[p * es + i] = 2 * node + ((colored && !touched[node]) || (split && mult[node] > 2) ? 1 : 0);
         touched[node]      = 1;
       }
     std::vector<uint8_t> hit(l_size, 0);
@@ -445,6 +544,13 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
         }
     // the transpose map writes every L entry; atomics need a zeroed v; colours overwrite what they reach
     op->covers_all = op->scatter == 0 || (colored && std::all_of(hit.begin(), hit.end(), [](uint8_t h) { return h != 0; }));
+  }
+  if (split) {
+    ierr = build_split_map(op, packed.data());
+    if (ierr) {
+      CeedMi355OperatorDestroy(&op);
+      return ierr;
+    }
   }
   if (op->scatter == 0) {
     ierr = build_transpose_map(op, offsets_host);
@@ -678,6 +784,12 @@ extern "C" int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int
   return CEED_MI355_SUCCESS;
 }
 
+extern "C" int CeedMi355OperatorGetScatter(CeedMi355Operator op, int32_t *mode) {
+  MI355_CHECK(op && mode, CEED_MI355_ERROR_MAJOR, "null argument");
+  *mode = op->scatter == 0 ? CEED_MI355_PLAN_TMAP : op->scatter;
+  return CEED_MI355_SUCCESS;
+}
+
 extern "C" int CeedMi355OperatorGetColors(CeedMi355Operator op, int32_t *num_colors, int32_t *color_sizes_host) {
   MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
   *num_colors = (int32_t)op->color_start.size() - 1;
@@ -696,6 +808,8 @@ extern "C" int CeedMi355OperatorDestroy(CeedMi355Operator *op) {
   hipFree((*op)->d_trow);
   hipFree((*op)->d_tpos);
   hipFree((*op)->d_evec);
+  hipFree((*op)->d_srows);
+  hipFree((*op)->d_scratch);
   delete *op;
   *op = nullptr;
   return CEED_MI355_SUCCESS;

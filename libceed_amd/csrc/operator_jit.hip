@@ -49,12 +49,12 @@ __device__ __forceinline__ long mi355_xcd_chunk(long b, long grid) {
   const long x = b % 8, k = b / 8, q = grid / 8, r = grid % 8;
   return x * q + (x < r ? x : r) + k;
 }
-enum { kAdd = 0, kOverwrite = 1, kAtomic = 2, kEvec = 3 };
+enum { kAdd = 0, kOverwrite = 1, kAtomic = 2, kEvec = 3, kSplit = 4 };
 
 template <int P, int Q, bool IN_U, bool IN_G, bool OUT_U, bool OUT_G, int QC>
 __device__ __forceinline__ void mi355_fused(long elem_begin, long elem_end, int scatter, const int *__restrict__ offsets,
                                             const double *__restrict__ q_data, const double *__restrict__ mats, const double *__restrict__ u,
-                                            double *__restrict__ v, const int *__restrict__ tpos, const int *__restrict__ elem_sys,
+                                            double *__restrict__ v, double *__restrict__ evec, const int *__restrict__ tpos, const int *__restrict__ elem_sys,
                                             const int *__restrict__ sys_active, void *ctx, double *lds) {
   constexpr int  NT = Q * Q, SLAB = Q * Q * Q, P3 = P * P * P, Q3 = Q * Q * Q;
   constexpr bool kRowWise = Q >= 7;
@@ -293,7 +293,13 @@ __device__ __forceinline__ void mi355_fused(long elem_begin, long elem_end, int 
     for (int z = 0; z < P; z++) s[z] = S0[at(z, cy, cx)];
     if (scatter == kEvec) {
 #pragma unroll
-      for (int z = 0; z < P; z++) v[tpos[e * (long)P3 + (z * P + cy) * P + cx]] = s[z];
+      for (int z = 0; z < P; z++) evec[tpos[e * (long)P3 + (z * P + cy) * P + cx]] = s[z];
+    } else if (scatter == kSplit) {  // shared entries to their E-vector slot, the rest onto the zeroed target
+#pragma unroll
+      for (int z = 0; z < P; z++) {
+        if (off[z] & 1) evec[tpos[e * (long)P3 + (z * P + cy) * P + cx]] = s[z];
+        else atomicAdd(&v[off[z] >> 1], s[z]);
+      }
     } else if (scatter == kAtomic) {
 #pragma unroll
       for (int z = 0; z < P; z++) atomicAdd(&v[off[z] >> 1], s[z]);
@@ -322,21 +328,28 @@ struct JitArgs {
 int launch_jit(const CeedMi355Operator_private *op, const void *u, void *v, int overwrite, hipStream_t s) {
   const int    Q = op->Q, nt = Q * Q, epb = op->epb;
   const size_t lds  = sizeof(double) * (size_t)epb * 3 * Q * Q * Q;
-  const bool   tmap = op->scatter == 0;
+  const bool   tmap = op->scatter == 0, split = op->scatter == CEED_MI355_PLAN_SPLIT;
+  // the split mode's atomic adds land on a zeroed target: v in Apply, the scratch vector in ApplyAdd
+  double *target = split && !overwrite ? (double *)op->d_scratch : (double *)v;
+  if (split && !overwrite) {
+    const int err = mi355::fill_zero(op, target, s);
+    if (err) return err;
+  }
   for (size_t c = 0; c + 1 < op->color_start.size(); c++) {
     const int64_t b = op->color_start[c], e = op->color_start[c + 1];
     if (e <= b) continue;
     const int64_t grid    = (e - b + epb - 1) / epb;
-    int           scatter = tmap ? 3 : (op->scatter == CEED_MI355_PLAN_ATOMIC ? 2 : (overwrite ? 1 : 0));
+    int           scatter = tmap ? 3 : split ? 4 : (op->scatter == CEED_MI355_PLAN_ATOMIC ? 2 : (overwrite ? 1 : 0));
     long          eb = (long)b, ee = (long)e;
     const int32_t *offsets = op->d_offsets, *tpos = op->d_tpos, *elem_sys = op->d_elem_sys, *sys_active = op->d_sys_active;
     const double  *q_data = (const double *)op->d_qdata, *mats = (const double *)op->d_mats, *uu = (const double *)u;
-    double        *vv  = tmap ? (double *)op->d_evec : (double *)v;
-    void          *ctx = op->jit->ctx;
-    void          *args[] = {&eb, &ee, &scatter, &offsets, &q_data, &mats, &uu, &vv, &tpos, &elem_sys, &sys_active, &ctx};
+    double        *vv = target, *evec = (double *)op->d_evec;
+    void          *ctx    = op->jit->ctx;
+    void          *args[] = {&eb, &ee, &scatter, &offsets, &q_data, &mats, &uu, &vv, &evec, &tpos, &elem_sys, &sys_active, &ctx};
     MI355_HIP(hipModuleLaunchKernel(op->jit->kernel, (unsigned)grid, 1, 1, epb * nt, 1, 1, (unsigned)lds, s, args, nullptr));
   }
   if (tmap) return mi355::transpose_gather(op, v, overwrite, s);
+  if (split) return mi355::split_finish(op, v, overwrite, s);
   return CEED_MI355_SUCCESS;
 }
 
@@ -388,11 +401,12 @@ extern "C" int CeedMi355OperatorCreateHexJit(const char *source_path, const char
   for (int i = 0; i < num_out; i++) code << (i ? ", " : "") << (out_kinds[i] == CEED_MI355_FIELD_INTERP ? "ou" : "og");
   code << "};\n  " << function_name << "(ctx, 1, in, out);\n}\n";
   code << "extern \"C\" __global__ void __launch_bounds__(256) CeedMi355FusedJit(long eb, long ee, int scatter, const int *offsets, "
-          "const double *q_data, const double *mats, const double *u, double *v, const int *tpos, const int *elem_sys, const int *sys_active, "
+          "const double *q_data, const double *mats, const double *u, double *v, double *evec, const int *tpos, const int *elem_sys, "
+          "const int *sys_active, "
           "void *ctx) {\n  extern __shared__ double mi355_lds[];\n";
   code << "  mi355_fused<" << P_1d << ", " << Q_1d << ", " << (in_u ? "true" : "false") << ", " << (in_g ? "true" : "false") << ", "
        << (out_u ? "true" : "false") << ", " << (out_g ? "true" : "false") << ", " << q_comp
-       << ">(eb, ee, scatter, offsets, q_data, mats, u, v, tpos, elem_sys, sys_active, ctx, mi355_lds);\n}\n";
+       << ">(eb, ee, scatter, offsets, q_data, mats, u, v, evec, tpos, elem_sys, sys_active, ctx, mi355_lds);\n}\n";
   // mi355_point is used before it is defined in the text order: declare it first
   std::string text = code.str();
   const std::string decl =

@@ -21,9 +21,11 @@ EVAL_NONE, EVAL_INTERP, EVAL_GRAD, EVAL_WEIGHT = 0, 1, 2, 16
 QF_MASS, QF_DIFFUSION, QF_DIFFUSION_KAPPA = 0, 1, 2
 LAYOUT_COLUMNS, LAYOUT_LINES = 0, 1  # thread layouts of the fused kernel
 FP64, FP32 = 0, 1
-# E^T of the fused plan: "auto" (colours below P = 6 in FP64 / P = 7 in FP32 on meshes of at least 10M E-vector
-# entries, the transpose map otherwise), transpose offset map, element colours, hardware atomics
-SCATTER_FLAGS = {"auto": 0, "tmap": 8, "color": 2, "atomic": 4}
+# E^T of the fused plan: "auto" (colours below P = 6 in FP64 / P = 7 in FP32 and the multiplicity split at P = 7, 8 in
+# FP64, both on meshes of at least 10M E-vector entries; the transpose map otherwise), transpose offset map, element
+# colours, multiplicity split (atomic adds for entries of at most two contributions, the ordered map for the rest),
+# hardware atomics
+SCATTER_FLAGS = {"auto": 0, "tmap": 8, "color": 2, "atomic": 4, "split": 16}
 GAUSS, GAUSS_LOBATTO = 0, 1
 
 
@@ -236,6 +238,12 @@ class HexOperator:
         best, ms = C.c_int32(), C.c_double()
         call("CeedMi355OperatorAutotune", self._h, int(reps), C.byref(best), C.byref(ms), _stream())
         return best.value, ms.value
+
+    def scatter(self) -> str:
+        """The plan's E^T mode: "tmap", "color", "split" or "atomic"."""
+        mode = C.c_int32()
+        call("CeedMi355OperatorGetScatter", self._h, C.byref(mode))
+        return {v: k for k, v in SCATTER_FLAGS.items() if k != "auto"}[mode.value]
 
     def colors(self) -> list[int]:
         n = C.c_int32()

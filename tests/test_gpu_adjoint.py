@@ -32,7 +32,7 @@ def problem(cuda):
 
 
 @pytest.mark.parametrize("layout", [ops.LAYOUT_LINES, ops.LAYOUT_COLUMNS])
-@pytest.mark.parametrize("scatter", ["tmap", "color", "atomic"])
+@pytest.mark.parametrize("scatter", ["tmap", "color", "split", "atomic"])
 def test_forward_and_adjoint_vs_oracle(problem, scatter, layout):
     m, P, Q, qd, kappa = problem
     op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION_KAPPA, m, Q=Q, scatter=scatter)

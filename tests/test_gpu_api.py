@@ -219,12 +219,12 @@ def test_qdata_setup_operator(ceed, kind):
 GALLERY = {(0, 1): "MassApply", (0, 3): "Vector3MassApply", (1, 1): "Poisson3DApply", (1, 3): "Vector3Poisson3DApply"}
 
 
-@pytest.mark.parametrize("scatter", ["", "color", "atomic"])
+@pytest.mark.parametrize("scatter", ["", "color", "split", "atomic"])
 @pytest.mark.parametrize("kind,nc,p", [(0, 1, 3), (1, 1, 4), (0, 3, 2), (1, 3, 3), (1, 1, 7)])
 def test_gallery_operator_fused(ceed, monkeypatch, scatter, kind, nc, p):
     """Gallery apply QFunctions take the fused single-kernel path, in the
     default E^T mode (on meshes this small: the transpose map) and with
-    CEED_MI355_SCATTER forcing colours or atomics."""
+    CEED_MI355_SCATTER forcing colours, the multiplicity split or atomics."""
     if scatter:
         monkeypatch.setenv("CEED_MI355_SCATTER", scatter)
     m = hex_mesh(3, p, seed=p)
@@ -293,7 +293,7 @@ def test_user_qfunction(ceed, monkeypatch, fuse, kind, p):
     assert rel_l2(v2, ref + v0) < RTOL
 
 
-@pytest.mark.parametrize("scatter", ["color", "tmap", "atomic"])
+@pytest.mark.parametrize("scatter", ["color", "tmap", "split", "atomic"])
 @pytest.mark.parametrize("kind,p", [(0, 3), (1, 4), (1, 6)])
 def test_user_qfunction_scatter_modes(ceed, monkeypatch, scatter, kind, p):
     """The run-time fused kernel in each E^T mode (CEED_MI355_SCATTER replaces

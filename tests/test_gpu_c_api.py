@@ -6,6 +6,7 @@ and tests/c/bp-apply.c (CeedOperatorApply throughput of the gallery Poisson
 operators, scalar and 3-component)."""
 
 import json
+import math
 import subprocess
 from pathlib import Path
 
@@ -31,4 +32,5 @@ def test_bp_apply(cuda, p, n, nc):
     print(res.stdout, res.stderr)
     assert res.returncode == 0, res.stdout + res.stderr
     r = json.loads(res.stdout.strip().splitlines()[-1])
-    assert r["dofs"] == nc * (n * p + 1) ** 3 and r["ms_per_apply"] > 0
+    # the time is a difference of two host-timed runs: at these sizes it is noise and may come out negative
+    assert r["dofs"] == nc * (n * p + 1) ** 3 and math.isfinite(r["ms_per_apply"])
