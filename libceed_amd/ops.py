@@ -239,7 +239,7 @@ class HexOperator:
         call("CeedMi355OperatorAutotune", self._h, int(reps), C.byref(best), C.byref(ms), _stream())
         return best.value, ms.value
 
-    def scatter(self) -> str:
+    def scatter_mode(self) -> str:
         """The plan's E^T mode: "tmap", "color", "split" or "atomic"."""
         mode = C.c_int32()
         call("CeedMi355OperatorGetScatter", self._h, C.byref(mode))

@@ -247,6 +247,7 @@ def test_split_scatter(cuda, kind, nc, n, p, Q, fp32, layout):
     out = {}
     for det in ("tmap", "split"):
         op = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, dtype=dtype, scatter=det)
+        assert op.scatter_mode() == det
         if kappa is not None:
             op.set_kappa(dev(kappa))
         op.set_layout(layout, 0)
