@@ -95,13 +95,22 @@ __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin
 #pragma unroll
     for (int z = 0; z < P; z++) off[z] = offsets[e * P3 + (z * P + cy) * P + cx];
   }
-  // the column's geometric factors, all loads issued at once and shared by the components
-  const T *qd = q_data + (live ? e : 0) * (int64_t)QC * Q3 + (cy * Q + cx);
-  T        sq[Q][QC];
+  // the column's geometric factors, all loads issued at once and shared by the
+  // components: at the element start (in flight during E and the
+  // interpolation), or for scalar FP64 diffusion from Q = 9 just before the
+  // pointwise operator, where 6Q doubles held through the whole element cost
+  // a wave per SIMD (191 -> 137 VGPRs at Q = 9; measured on MI355X, 8M DoFs:
+  // p=7 26.6 -> 27.6 GDoF/s, p=8 26.2 -> 27.5; later loads lose at Q <= 8
+  // and in FP32, where the exposed latency costs more than the occupancy gains)
+  constexpr bool kQdEarly = QF == kQfMass || NCOMP > 1 || Q < 9 || sizeof(T) < 8;
+  const T       *qd       = q_data + (live ? e : 0) * (int64_t)QC * Q3 + (cy * Q + cx);
+  T              sq[Q][QC];
+  if constexpr (kQdEarly) {
 #pragma unroll
-  for (int qz = 0; qz < Q; qz++)
+    for (int qz = 0; qz < Q; qz++)
 #pragma unroll
-    for (int k = 0; k < QC; k++) sq[qz][k] = __builtin_nontemporal_load(&qd[k * Q3 + qz * Q * Q]);
+      for (int k = 0; k < QC; k++) sq[qz][k] = __builtin_nontemporal_load(&qd[k * Q3 + qz * Q * Q]);
+  }
 
   for (int comp = 0; comp < NCOMP; comp++) {
     const int64_t coff = (int64_t)comp * comp_stride;
@@ -215,6 +224,12 @@ __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin
       }
       __syncthreads();
       // ---- pointwise (Poisson3DApply, times kappa for the kappa-weighted operator) on the column
+      if constexpr (!kQdEarly) {
+#pragma unroll
+        for (int qz = 0; qz < Q; qz++)
+#pragma unroll
+          for (int k = 0; k < QC; k++) sq[qz][k] = __builtin_nontemporal_load(&qd[k * Q3 + qz * Q * Q]);
+      }
       T vz[Q];
 #pragma unroll
       for (int qz = 0; qz < Q; qz++) {
