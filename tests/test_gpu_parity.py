@@ -326,3 +326,33 @@ def test_operator_invariants_large(cuda, p):
     A.apply(a, Aa)
     A.apply(b, Ab)
     assert abs(torch.dot(a, Ab).item() - torch.dot(b, Aa).item()) < 1e-10 * abs(torch.dot(a, Ab).item())
+
+
+@pytest.mark.parametrize("p,mode", [(4, "color"), (6, "split"), (7, "split")])
+def test_default_modes_full_size(cuda, p, mode):
+    """At ~8M DoFs the size-based default E^T is colours (p <= 4) or the multiplicity split (p = 6, 7): the
+    split's Apply is bitwise equal to the transpose map's there as on small meshes, colours agree with it to
+    rounding, and A 1 = 0 and symmetry hold for the default plan."""
+    from libceed_amd.mesh import mesh_size_for_dofs
+
+    m = hex_mesh(mesh_size_for_dofs(p, 8e6), p, seed=31)
+    Q = p + 2
+    A = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q)
+    assert A.scatter_mode() == mode
+    T = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q, scatter="tmap")
+    g = torch.Generator(device="cuda").manual_seed(p)
+    a = torch.rand(m.num_nodes, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    b = torch.rand(m.num_nodes, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    Aa, Ta, Ab = torch.empty_like(a), torch.empty_like(a), torch.empty_like(a)
+    A.apply(a, Aa)
+    T.apply(a, Ta)
+    print("p", p, mode, "dofs", m.num_nodes, flush=True)
+    if mode == "split":
+        assert torch.equal(Aa, Ta)
+    else:
+        assert (torch.linalg.norm(Aa - Ta) / torch.linalg.norm(Ta)).item() < 1e-13
+    A.apply(b, Ab)
+    assert abs(torch.dot(a, Ab).item() - torch.dot(b, Aa).item()) < 1e-10 * abs(torch.dot(a, Ab).item())
+    one = torch.ones_like(a)
+    A.apply(one, Ab)
+    assert Ab.abs().max().item() < 1e-10
