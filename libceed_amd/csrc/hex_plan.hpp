@@ -61,7 +61,7 @@ struct CeedMi355Operator_private {
   int     epb;
   int     layout     = 0;      // mi355::kLayoutColumns or kLayoutLines
   bool    covers_all = false;  // every L entry is reached by some element: Apply needs no zero fill
-  int     scatter    = 0;      // CEED_MI355_PLAN_* E^T mode: 0 transpose map, COLOR, ATOMIC
+  int     scatter    = 0;      // CEED_MI355_PLAN_* E^T mode: 0 transpose map, COLOR, SPLIT, ATOMIC
   // transpose offset map (scatter == 0): contribution (elem, comp, node) is
   // stored at d_evec[d_tpos[(elem * num_comp + comp) * P^3 + node]], so that
   // L entry t owns the contiguous run d_evec[d_trow[t] .. d_trow[t+1]), in
