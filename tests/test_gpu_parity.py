@@ -356,3 +356,52 @@ def test_default_modes_full_size(cuda, p, mode):
     one = torch.ones_like(a)
     A.apply(one, Ab)
     assert Ab.abs().max().item() < 1e-10
+
+
+@pytest.mark.parametrize("scatter", ["tmap", "color", "split", "atomic"])
+@pytest.mark.parametrize("kind,nc,p", [(1, 1, 3), (0, 1, 4), (1, 3, 2)])
+def test_operator_unstructured_numbering(cuda, scatter, kind, nc, p):
+    """Nodes renumbered by a random permutation into an L-vector with unused entries, elements in random order
+    (so colours, maps and multiplicities come from a numbering with no structure): Apply matches the oracle and
+    leaves the unreached entries 0; ApplyAdd leaves them as they were."""
+    m = hex_mesh(3, p, seed=13 + p)
+    rng = np.random.default_rng(100 + p)
+    P, Q, E = p + 1, p + 2, m.num_elem
+    stride = m.num_nodes + 11                                    # per-component range with 11 unused entries
+    perm = rng.permutation(stride)[: m.num_nodes].astype(np.int32)
+    order = rng.permutation(E)
+    offsets = perm[m.offsets.reshape(E, -1)[order]].reshape(-1)
+    qc = 1 if kind == 0 else 6
+    qd = oracle.hex_build_qdata(kind, m, Q).reshape(E, qc * Q ** 3)[order].reshape(-1)
+    l_size = nc * stride
+    u = rng.uniform(-1, 1, l_size)
+    v_ref = oracle.hex_apply(kind, nc, E, P, Q, offsets, stride, qd, u)
+    unused = np.ones(l_size, dtype=bool)
+    for k in range(nc):
+        unused[perm + k * stride] = False
+    op = ops.HexOperator(kind, offsets, E, P, Q, l_size, num_comp=nc, comp_stride=stride, scatter=scatter)
+    assert op.scatter_mode() == scatter
+    op.set_qdata(dev(qd))
+    v = torch.full((l_size,), 3.0, dtype=torch.float64, device="cuda")
+    op.apply(dev(u), v)
+    vh = host(v)
+    assert np.all(vh[unused] == 0.0)
+    assert rel_l2(vh, v_ref) < RTOL64
+    w0 = rng.uniform(-1, 1, l_size)
+    w = dev(w0)
+    op.apply_add(dev(u), w)
+    wh = host(w)
+    assert np.array_equal(wh[unused], w0[unused])
+    assert rel_l2(wh, w0 + v_ref) < RTOL64
+
+
+@pytest.mark.parametrize("scatter", ["tmap", "color", "split", "atomic"])
+def test_operator_no_elements(cuda, scatter):
+    """A plan with no elements: Apply gives zeros, ApplyAdd leaves v unchanged."""
+    op = ops.HexOperator(ops.QF_DIFFUSION, np.zeros(0, dtype=np.int32), 0, 3, 4, 10, scatter=scatter)
+    v = torch.full((10,), 3.0, dtype=torch.float64, device="cuda")
+    op.apply(torch.ones(10, dtype=torch.float64, device="cuda"), v)
+    assert torch.count_nonzero(v).item() == 0
+    w = torch.full((10,), 2.0, dtype=torch.float64, device="cuda")
+    op.apply_add(torch.ones(10, dtype=torch.float64, device="cuda"), w)
+    assert torch.all(w == 2.0).item()
