@@ -48,6 +48,20 @@ __device__ __forceinline__ const_ptr<T> phase_ptr(const T *p) {
   return (const_ptr<T>)p;
 }
 
+// Lanes 2m and 2m+1 of a wave hold the columns (cx, cy) and (cx+1, cy), cx
+// even: each loads q_data for both columns, half of the components each, with
+// 16-byte loads (half the load instructions), and they swap halves with a DPP
+// quad permutation [1, 0, 3, 2] when the values are used.
+template <typename T>
+using line_v2 = T __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ double pair_swap(double x) {
+  const long long b  = __double_as_longlong(x);
+  const int       lo = __builtin_amdgcn_mov_dpp((int)b, 0xB1, 0xF, 0xF, false);
+  const int       hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 template <int QF>
 struct LineSlabs {
   static constexpr int kSlabs = QF == kQfMass ? 2 : 3;
@@ -103,14 +117,30 @@ __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin
   // p=7 26.6 -> 27.6 GDoF/s, p=8 26.2 -> 27.5; later loads lose at Q <= 8
   // and in FP32, where the exposed latency costs more than the occupancy gains)
   constexpr bool kQdEarly = QF == kQfMass || NCOMP > 1 || Q < 9 || sizeof(T) < 8;
+  // paired 16-byte loads with the early loads of FP64 scalar diffusion, Q even
+  // (interleaved A/B on MI355X, 8M DoFs: p=4 +0.7% with colours, +3.7% with
+  // the map, p=6 +0-4% with the split; with the late loads at Q = 10 the swap
+  // in the pointwise loop cost 15%; FP32 pairs of 8-byte loads were 3-5% slower)
+  constexpr bool kPairQd = kQdEarly && sizeof(T) == 8 && QF == kQfDiffusion && NCOMP == 1 && Q % 2 == 0;
+  constexpr int  QH      = QC / 2;
   const T       *qd       = q_data + (live ? e : 0) * (int64_t)QC * Q3 + (cy * Q + cx);
-  T              sq[Q][QC];
-  if constexpr (kQdEarly) {
+  T              sq[kPairQd ? 1 : Q][QC];
+  line_v2<T>     sp[kPairQd ? Q : 1][kPairQd ? QH : 1];
+  auto           load_q = [&]() {
+    if constexpr (kPairQd) {  // 16-byte aligned: Q even, so (cy * Q + cx - odd) and every stride are even
+      const T *pb = q_data + (live ? e : 0) * (int64_t)QC * Q3 + (cy * Q + (cx & ~1)) + ((cx & 1) ? QH : 0) * Q3;
 #pragma unroll
-    for (int qz = 0; qz < Q; qz++)
+      for (int qz = 0; qz < Q; qz++)
 #pragma unroll
-      for (int k = 0; k < QC; k++) sq[qz][k] = __builtin_nontemporal_load(&qd[k * Q3 + qz * Q * Q]);
-  }
+        for (int kk = 0; kk < QH; kk++) sp[qz][kk] = __builtin_nontemporal_load((const line_v2<T> *)(pb + kk * Q3 + qz * Q * Q));
+    } else {
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++)
+#pragma unroll
+        for (int k = 0; k < QC; k++) sq[qz][k] = __builtin_nontemporal_load(&qd[k * Q3 + qz * Q * Q]);
+    }
+  };
+  if constexpr (kQdEarly) load_q();
 
   for (int comp = 0; comp < NCOMP; comp++) {
     const int64_t coff = (int64_t)comp * comp_stride;
@@ -224,17 +254,25 @@ __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin
       }
       __syncthreads();
       // ---- pointwise (Poisson3DApply, times kappa for the kappa-weighted operator) on the column
-      if constexpr (!kQdEarly) {
-#pragma unroll
-        for (int qz = 0; qz < Q; qz++)
-#pragma unroll
-          for (int k = 0; k < QC; k++) sq[qz][k] = __builtin_nontemporal_load(&qd[k * Q3 + qz * Q * Q]);
-      }
+      if constexpr (!kQdEarly) load_q();
       T vz[Q];
 #pragma unroll
       for (int qz = 0; qz < Q; qz++) {
         T vx, vy, vzz;
-        poisson_regs<T>(sq[qz], S2[at(qz, cy, cx)], S0[at(qz, cy, cx)], dz[qz], vx, vy, vzz);
+        if constexpr (kPairQd) {  // this column's half from the own load, the other half from the partner lane
+          const bool odd = cx & 1;
+          T          s6[QC];
+#pragma unroll
+          for (int kk = 0; kk < QH; kk++) {
+            const line_v2<T> d = sp[qz][kk];
+            const T       keep = odd ? d.y : d.x, got = pair_swap(odd ? d.x : d.y);
+            s6[kk]             = odd ? got : keep;
+            s6[QH + kk]        = odd ? keep : got;
+          }
+          poisson_regs<T>(s6, S2[at(qz, cy, cx)], S0[at(qz, cy, cx)], dz[qz], vx, vy, vzz);
+        } else {
+          poisson_regs<T>(sq[qz], S2[at(qz, cy, cx)], S0[at(qz, cy, cx)], dz[qz], vx, vy, vzz);
+        }
         if constexpr (QF == kQfDiffusionKappa) {
           vx *= sq[qz][6];
           vy *= sq[qz][6];
