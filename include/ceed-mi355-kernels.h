@@ -50,7 +50,7 @@ enum { CEED_MI355_QF_MASS = 0, CEED_MI355_QF_DIFFUSION = 1, CEED_MI355_QF_DIFFUS
 enum { CEED_MI355_FIELD_INTERP = 0, CEED_MI355_FIELD_GRAD = 1, CEED_MI355_FIELD_PASSIVE = 2 };
 
 /* E^T mode flags of CeedMi355OperatorCreateHex (0: transpose map for P >= 6, colours below). */
-enum { CEED_MI355_PLAN_COLOR = 2, CEED_MI355_PLAN_ATOMIC = 4, CEED_MI355_PLAN_TMAP = 8, CEED_MI355_PLAN_SPLIT = 16 };
+enum { CEED_MI355_PLAN_COLOR = 2, CEED_MI355_PLAN_ATOMIC = 4, CEED_MI355_PLAN_TMAP = 8, CEED_MI355_PLAN_SPLIT = 16, CEED_MI355_PLAN_BRICK = 32 };
 
 /* Geometric-factor builders (gallery Mass3DBuild / Poisson3DBuild). */
 enum { CEED_MI355_BUILD_MASS = 0, CEED_MI355_BUILD_DIFFUSION = 1 };

@@ -62,6 +62,10 @@ __global__ void __launch_bounds__(128) hex_diag_kernel(int64_t elem_begin, int64
     }
     const int32_t o = offsets[e * P3 + i];
     for (int k = 0; k < num_comp; k++) {
+      if (scatter == kScatterElem) {  // element-major E-vector (brick plans)
+        evec[(e * num_comp + k) * (int64_t)P3 + i] = acc;
+        continue;
+      }
       if (scatter == kScatterEvec || (scatter == kScatterSplit && (o & 1))) {  // L-entry-sorted E-vector
         evec[tpos[(e * num_comp + k) * (int64_t)P3 + i]] = acc;
         continue;
@@ -88,6 +92,18 @@ int launch_diag(const CeedMi355Operator_private *op, void *v, int add, hipStream
   const size_t lds  = sizeof(T) * (size_t)(2 * Q * P + op->q_comp * Q * Q * Q);
   const bool   tmap = op->scatter == 0;
   const bool   split = op->scatter == CEED_MI355_PLAN_SPLIT;
+  const bool   brick = op->scatter == CEED_MI355_PLAN_BRICK;
+  if (brick) {  // element diagonals into the element-major E-vector, then the brick E^T
+    if (!add && !op->covers_all) {
+      diag_zero<T><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, (T *)v);
+      MI355_LAUNCH_CHECK();
+    }
+    hex_diag_kernel<P, Q, T><<<(unsigned)op->num_elem, 128, lds, s>>>(0, op->num_elem, kScatterElem, op->qf, op->num_comp, op->d_offsets,
+                                                                     op->comp_stride, (const T *)op->d_qdata, op->q_comp,
+                                                                     (const T *)op->d_mats, (T *)v, (T *)op->d_evec, op->d_tpos);
+    MI355_LAUNCH_CHECK();
+    return brick_evec(op, op->d_evec, v, !add, s);
+  }
   // the split mode's atomic adds land on a zeroed target: v, or with add the scratch vector
   T *target = split && add ? (T *)op->d_scratch : (T *)v;
   if ((!add && !tmap && !op->covers_all) || (split && add)) {
@@ -131,6 +147,8 @@ extern "C" int CeedMi355OperatorAssembleDiagonal(CeedMi355Operator op, void *dia
   MI355_CHECK(op->qf == mi355::kQfMass || op->qf == mi355::kQfDiffusion || op->qf == mi355::kQfDiffusionKappa, CEED_MI355_ERROR_UNSUPPORTED,
               "diagonal assembly needs a mass or diffusion plan");
   if (op->num_elem == 0) return CEED_MI355_SUCCESS;
+  if (op->scatter == CEED_MI355_PLAN_BRICK && !op->d_evec)  // element-major E-vector of the brick plan's diagonal, made once
+    MI355_HIP(hipMalloc(&op->d_evec, (size_t)op->scalar_bytes * ((size_t)op->num_elem * op->num_comp * op->P * op->P * op->P + 1)));
   auto fn = op->scalar_type == CEED_MI355_FP64 ? mi355::pick_diag<double>(op->P, op->Q) : mi355::pick_diag<float>(op->P, op->Q);
   MI355_CHECK(fn, CEED_MI355_ERROR_UNSUPPORTED, "no diagonal kernel for P=%d Q=%d", op->P, op->Q);
   return fn(op, diag, add, (hipStream_t)stream);

@@ -26,8 +26,9 @@ enum { kQfMass = 0, kQfDiffusion = 1, kQfDiffusionKappa = 2, kQfJit = 3 };
 // E^T modes: colour-exclusive read-modify-write (ApplyAdd), the same with the
 // first-touch overwrite (Apply), hardware atomics (one launch, any order), or
 // store of each contribution into an E-vector sorted by L entry (one launch;
-// a separate gather sums each L entry's contiguous run of contributions).
-enum { kScatterAdd = 0, kScatterOverwrite = 1, kScatterAtomic = 2, kScatterEvec = 3, kScatterSplit = 4 };
+// a separate gather sums each L entry's contiguous run of contributions), or
+// (diagonal assembly of brick plans) store into an element-major E-vector.
+enum { kScatterAdd = 0, kScatterOverwrite = 1, kScatterAtomic = 2, kScatterEvec = 3, kScatterSplit = 4, kScatterElem = 5 };
 
 template <int QF>
 struct QfTraits;

@@ -6,6 +6,7 @@
 
 #include <vector>
 
+#include "hex_brick.hpp"
 #include "hex_fused.hpp"
 #include "hex_lines.hpp"
 #include "mi355_common.h"
@@ -21,6 +22,12 @@ int transpose_gather(const CeedMi355Operator_private *op, void *v, int overwrite
 // v; ApplyAdd adds the scratch target and the shared sums into v.
 int split_finish(const CeedMi355Operator_private *op, void *v, int overwrite, hipStream_t s);
 int fill_zero(const CeedMi355Operator_private *op, void *x, hipStream_t s);
+// The brick mode's second step: v (=|+=) the partial sums of each brick-surface
+// node, in ascending brick order (operator_brick.hip).
+int brick_gather(const CeedMi355Operator_private *op, void *v, int overwrite, hipStream_t s);
+// v (=|+=) the brick E^T of the element-major E-vector evec[(elem * num_comp +
+// comp) * P^3 + node] in packed order: brick sums, then brick_gather.
+int brick_evec(const CeedMi355Operator_private *op, const void *evec, void *v, int overwrite, hipStream_t s);
 
 // Kernel launcher for one (P, Q, num_comp, QF, scalar type); nullptr when no instantiation exists.
 ApplyFn pick_apply_f64(int P, int Q, int num_comp, int qf);
@@ -44,6 +51,20 @@ int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes, int layout);
 }  // namespace mi355
 
 namespace mi355 {
+// Bricks of face-sharing elements for the brick E^T (operator_brick.hip):
+// elements grouped into up to bx * by * bz slots, the packed element order
+// (brick by brick, slots ascending) in perm_out, and the device arrays of
+// the plan (node lists, partial slots, the gather's map).  Returns
+// CEED_MI355_ERROR_UNSUPPORTED (and leaves the plan as it was) where the
+// mesh does not form bricks (an element repeats a node).
+int build_bricks(CeedMi355Operator_private *op, const int32_t *offsets, int bx, int by, int bz, std::vector<int32_t> &perm_out);
+// The host side of build_bricks, without device memory.
+struct BrickLayout {
+  int                  shape[3] = {1, 1, 1}, nb_nodes = 0;
+  int64_t              num_bricks = 0;
+  std::vector<int32_t> perm, start, mask, bnode, blist, brow, bslot;
+};
+int brick_layout(const int32_t *offsets, int64_t num_elem, int P, int num_comp, int bx, int by, int bz, BrickLayout &out);
 // A plan of CeedMi355OperatorCreateHex with the given kernel launcher and
 // geometric-factor components per point (operator_hex.hip).
 int create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_elem, int32_t P_1d, int32_t Q_1d, const double *interp_1d_host,
@@ -87,6 +108,22 @@ struct CeedMi355Operator_private {
   const int64_t *d_gather_chunk_begin = nullptr;
   const int32_t *d_gather_chunk_sys   = nullptr;
   int64_t        gather_num_chunks    = 0;
+  // brick mode (scatter == CEED_MI355_PLAN_BRICK): brick b owns the packed
+  // elements [d_brick_start[b], d_brick_start[b+1]) in the slots set in
+  // d_brick_mask[b] (slot s = (s % bx, s / bx % by, s / (bx * by))); its
+  // nb_nodes = (bx p + 1)(by p + 1)(bz p + 1) nodes, x fastest, are listed in
+  // d_bnode[b] (2 * node + 1 if an element outside the brick shares the node,
+  // -1 where no element of the brick lies); partial sums of the shared nodes
+  // go to d_partials[b][comp][local]; the gather sums, for each shared node
+  // d_blist[j], the partial slots d_bslot[d_brow[j] .. d_brow[j+1]) (each
+  // b * num_comp * nb_nodes + local), in ascending brick order
+  int      brick_shape[3] = {1, 1, 1};
+  int      brick_prefetch = 1;  // the brick kernel loads the next element's data during the current one
+  int64_t  num_bricks = 0, num_bound = 0;
+  int      nb_nodes = 0;
+  int32_t *d_brick_start = nullptr, *d_brick_mask = nullptr, *d_bnode = nullptr;
+  int32_t *d_blist = nullptr, *d_brow = nullptr, *d_bslot = nullptr;
+  void    *d_partials = nullptr;
   // colour c owns packed elements [color_start[c], color_start[c+1])
   std::vector<int64_t> color_start;
   std::vector<int32_t> perm;  // packed position -> caller's element index
@@ -106,6 +143,16 @@ int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v
   const int    epb   = op->epb;
   const bool   lines = op->layout == kLayoutLines;
   const size_t lds   = sizeof(T) * (size_t)(lines ? hex_lines_lds_entries<P, Q, QF>(epb) : hex_lds_entries<P, Q, QF>(epb));
+  if (op->scatter == CEED_MI355_PLAN_BRICK) {
+    if (op->num_bricks == 0) return CEED_MI355_SUCCESS;
+    const size_t blds = hex_brick_lds_bytes<P, Q, NC, QF, T>(op->nb_nodes);
+    auto         kern = op->brick_prefetch ? hex_apply_brick_kernel<P, Q, NC, QF, T, true> : hex_apply_brick_kernel<P, Q, NC, QF, T, false>;
+    kern<<<(unsigned)op->num_bricks, nt, blds, s>>>(op->brick_shape[0], op->brick_shape[1], op->nb_nodes, op->d_brick_start, op->d_brick_mask,
+                                                    op->d_bnode, op->comp_stride, (const T *)op->d_qdata, (const T *)op->d_mats, (const T *)u,
+                                                    (T *)v, (T *)op->d_partials, overwrite);
+    MI355_LAUNCH_CHECK();
+    return brick_gather(op, v, overwrite, s);
+  }
   const bool   tmap  = op->scatter == 0;
   const bool   split = op->scatter == CEED_MI355_PLAN_SPLIT;
   // the split mode's atomic adds land on a zeroed target: v in Apply (zeroed

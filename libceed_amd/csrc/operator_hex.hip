@@ -1,5 +1,6 @@
 // Fused hex operator plans: element colouring, colour-ordered data layout,
 // geometric factors on the GPU, and the apply entry points.
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -452,19 +453,37 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
   // at p = 6, 1.03x at p = 7, 8M DoFs.
   const int64_t evec_entries = (int64_t)num_elem * P_1d * P_1d * P_1d * num_comp;
   const bool    big          = evec_entries >= 10000000;
-  if (flags & CEED_MI355_PLAN_SPLIT) op->scatter = CEED_MI355_PLAN_SPLIT;
+  if (flags & CEED_MI355_PLAN_BRICK) op->scatter = CEED_MI355_PLAN_BRICK;
+  else if (flags & CEED_MI355_PLAN_SPLIT) op->scatter = CEED_MI355_PLAN_SPLIT;
   else if (flags & CEED_MI355_PLAN_ATOMIC) op->scatter = CEED_MI355_PLAN_ATOMIC;
   else if (flags & CEED_MI355_PLAN_COLOR) op->scatter = CEED_MI355_PLAN_COLOR;
   else if (flags & CEED_MI355_PLAN_TMAP) op->scatter = 0;
   else if (big && scalar_type == CEED_MI355_FP64 && (P_1d == 7 || P_1d == 8)) op->scatter = CEED_MI355_PLAN_SPLIT;
   else op->scatter = P_1d >= (scalar_type == CEED_MI355_FP32 ? 7 : 6) || !big ? 0 : CEED_MI355_PLAN_COLOR;
-  if (!(flags & (CEED_MI355_PLAN_ATOMIC | CEED_MI355_PLAN_COLOR | CEED_MI355_PLAN_TMAP | CEED_MI355_PLAN_SPLIT)))
-    if (const char *env = getenv("CEED_MI355_SCATTER")) {  // color | tmap | split | atomic replaces the default
+  const int explicit_modes = CEED_MI355_PLAN_ATOMIC | CEED_MI355_PLAN_COLOR | CEED_MI355_PLAN_TMAP | CEED_MI355_PLAN_SPLIT | CEED_MI355_PLAN_BRICK;
+  const int default_scatter = op->scatter;
+  if (!(flags & explicit_modes))
+    if (const char *env = getenv("CEED_MI355_SCATTER")) {  // color | tmap | split | atomic | brick replaces the default
       if (!strcmp(env, "color")) op->scatter = CEED_MI355_PLAN_COLOR;
       else if (!strcmp(env, "tmap")) op->scatter = 0;
       else if (!strcmp(env, "split")) op->scatter = CEED_MI355_PLAN_SPLIT;
       else if (!strcmp(env, "atomic")) op->scatter = CEED_MI355_PLAN_ATOMIC;
+      else if (!strcmp(env, "brick")) op->scatter = CEED_MI355_PLAN_BRICK;
+      else {
+        delete op;
+        MI355_CHECK(false, CEED_MI355_ERROR_BACKEND, "CEED_MI355_SCATTER=%s: expected color, tmap, split, atomic or brick", env);
+      }
     }
+  // the brick E^T exists for the gallery mass and diffusion kernels; a plan of
+  // another kind asked for it by flag is an error, by the environment it keeps
+  // its default
+  if (op->scatter == CEED_MI355_PLAN_BRICK && !(qf == kQfMass || qf == kQfDiffusion)) {
+    if (flags & CEED_MI355_PLAN_BRICK) {
+      delete op;
+      MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED, "the brick E^T supports mass and diffusion plans only");
+    }
+    op->scatter = default_scatter;
+  }
   // launch: the tuned (layout, elements per workgroup) of this kind of plan if the
   // autotuner has run on one in this process, else the line layout (faster
   // than the column layout at every order measured on MI355X, equal at p <= 2)
@@ -490,9 +509,47 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
   const int            es = P_1d * P_1d * P_1d;
   std::vector<int32_t> color;
   int                  num_colors = 0;
+  int                  ierr       = CEED_MI355_SUCCESS;
+  if (op->scatter == CEED_MI355_PLAN_BRICK) {
+    // bricks of face-sharing elements, one workgroup each; the shape by order
+    // (the brick's accumulator and node list share the CU's LDS with the slabs)
+    int shape[3] = {2, 2, 2};
+    if (P_1d >= 7) shape[1] = shape[2] = 1;
+    else if (P_1d == 6 || num_comp > 1) shape[2] = 1;
+    if (const char *env = getenv("CEED_MI355_BRICK")) {  // "bx,by,bz"
+      int a, b, c;
+      if (sscanf(env, "%d,%d,%d", &a, &b, &c) == 3 && a >= 1 && b >= 1 && c >= 1 && a * b * c <= 8) shape[0] = a, shape[1] = b, shape[2] = c;
+    }
+    if (const char *env = getenv("CEED_MI355_BRICK_PREFETCH")) op->brick_prefetch = atoi(env) != 0;
+    {  // the workgroup's LDS: slabs, accumulator, node list
+      const int64_t nb  = (int64_t)(shape[0] * (P_1d - 1) + 1) * (shape[1] * (P_1d - 1) + 1) * (shape[2] * (P_1d - 1) + 1);
+      const int64_t lds = op->scalar_bytes * ((qf == kQfMass ? 2 : 3) * (int64_t)Q_1d * Q_1d * Q_1d + num_comp * nb) + 4 * nb;
+      if (lds > mi355::kMaxLdsBytes) {
+        CeedMi355OperatorDestroy(&op);
+        MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED, "brick %dx%dx%d needs %lld bytes of LDS (at most %d)", shape[0], shape[1], shape[2],
+                    (long long)lds, mi355::kMaxLdsBytes);
+      }
+    }
+    ierr = num_elem > 0 ? mi355::build_bricks(op, offsets_host, shape[0], shape[1], shape[2], op->perm) : CEED_MI355_SUCCESS;
+    if (ierr == CEED_MI355_ERROR_UNSUPPORTED && !(flags & CEED_MI355_PLAN_BRICK)) {  // no bricks in this mesh: the default mode
+      for (void *d : {(void *)op->d_brick_start, (void *)op->d_brick_mask, (void *)op->d_bnode, (void *)op->d_blist, (void *)op->d_brow,
+                      (void *)op->d_bslot, op->d_partials})
+        (void)hipFree(d);
+      op->d_brick_start = op->d_brick_mask = op->d_bnode = op->d_blist = op->d_brow = op->d_bslot = nullptr;
+      op->d_partials = nullptr;
+      op->num_bricks = op->num_bound = 0;
+      op->scatter    = default_scatter == CEED_MI355_PLAN_BRICK ? 0 : default_scatter;
+      ierr           = CEED_MI355_SUCCESS;
+    }
+    if (ierr) {
+      CeedMi355OperatorDestroy(&op);
+      return ierr;
+    }
+  }
   const bool colored = op->scatter == CEED_MI355_PLAN_COLOR;
-  int ierr = colored ? color_elements(offsets_host, num_elem, es, num_comp, comp_stride, l_size, color, num_colors) : CEED_MI355_SUCCESS;
-  if (!colored) {  // one launch over all elements in element order
+  const bool bricks  = op->scatter == CEED_MI355_PLAN_BRICK;
+  if (colored) ierr = color_elements(offsets_host, num_elem, es, num_comp, comp_stride, l_size, color, num_colors);
+  else {  // one launch over all elements in element order (brick order for bricks)
     color.assign((size_t)num_elem, 0);
     num_colors = num_elem > 0 ? 1 : 0;
   }
@@ -500,12 +557,16 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
     delete op;
     return ierr;
   }
-  // packed order: by colour, ascending element index inside a colour
-  op->color_start.assign(num_colors + 1, 0);
-  for (int64_t e = 0; e < num_elem; e++) op->color_start[color[e] + 1]++;
-  for (int c = 0; c < num_colors; c++) op->color_start[c + 1] += op->color_start[c];
-  op->perm.resize(num_elem);
-  {
+  if (bricks) {
+    op->color_start.assign(2, 0);
+    op->color_start[1] = num_elem;
+    if (num_elem == 0) op->color_start.resize(1);
+  } else {
+    // packed order: by colour, ascending element index inside a colour
+    op->color_start.assign(num_colors + 1, 0);
+    for (int64_t e = 0; e < num_elem; e++) op->color_start[color[e] + 1]++;
+    for (int c = 0; c < num_colors; c++) op->color_start[c + 1] += op->color_start[c];
+    op->perm.resize(num_elem);
     std::vector<int64_t> cursor(op->color_start.begin(), op->color_start.end() - 1);
     for (int64_t e = 0; e < num_elem; e++) op->perm[cursor[color[e]]++] = (int32_t)e;
   }
@@ -518,7 +579,7 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
     int64_t n_nodes = 0;
     for (int64_t i = 0; i < (int64_t)num_elem * es; i++) n_nodes = std::max<int64_t>(n_nodes, (int64_t)offsets_host[i] + 1);
     if (n_nodes >= (1ll << 30)) {
-      delete op;
+      CeedMi355OperatorDestroy(&op);
       MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED, "more than 2^30 nodes");
     }
     std::vector<char>    touched(n_nodes, 0);
@@ -537,13 +598,13 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
         for (int k = 0; k < num_comp; k++) {
           uint8_t &h = hit[n + (int64_t)k * comp_stride];
           if (h) {
-            delete op;
+            CeedMi355OperatorDestroy(&op);
             MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED, "components overlap at L index %lld", (long long)(n + (int64_t)k * comp_stride));
           }
           h = 1;
         }
-    // the transpose map writes every L entry; atomics need a zeroed v; colours overwrite what they reach
-    op->covers_all = op->scatter == 0 || (colored && std::all_of(hit.begin(), hit.end(), [](uint8_t h) { return h != 0; }));
+    // the transpose map writes every L entry; atomics need a zeroed v; colours and bricks overwrite what they reach
+    op->covers_all = op->scatter == 0 || ((colored || bricks) && std::all_of(hit.begin(), hit.end(), [](uint8_t h) { return h != 0; }));
   }
   if (split) {
     ierr = build_split_map(op, packed.data());
@@ -810,6 +871,13 @@ extern "C" int CeedMi355OperatorDestroy(CeedMi355Operator *op) {
   hipFree((*op)->d_evec);
   hipFree((*op)->d_srows);
   hipFree((*op)->d_scratch);
+  hipFree((*op)->d_brick_start);
+  hipFree((*op)->d_brick_mask);
+  hipFree((*op)->d_bnode);
+  hipFree((*op)->d_blist);
+  hipFree((*op)->d_brow);
+  hipFree((*op)->d_bslot);
+  hipFree((*op)->d_partials);
   delete *op;
   *op = nullptr;
   return CEED_MI355_SUCCESS;

@@ -35,51 +35,79 @@ int CeedQFunctionContextGetCeed(CeedQFunctionContext ctx, Ceed *ceed) {
   return CeedReferenceCopy(ctx->ceed, ceed);
 }
 
+/* Data access: the backend stores the bytes (slots SetData, GetData, ...);
+ * the core keeps the size, the state (odd while a write access is out) and
+ * the reader count, as interface/ceed-qfunctioncontext.c:643-820 does. */
+int CeedQFunctionContextHasValidData(CeedQFunctionContext ctx, bool *has_valid_data) {
+  CeedCheck(ctx->HasValidData, ctx->ceed, CEED_ERROR_UNSUPPORTED, "backend does not report whether a context holds data");
+  return ctx->HasValidData(ctx, has_valid_data);
+}
+
+int CeedQFunctionContextHasBorrowedDataOfType(CeedQFunctionContext ctx, CeedMemType mem_type, bool *has_borrowed_data_of_type) {
+  CeedCheck(ctx->HasBorrowedDataOfType, ctx->ceed, CEED_ERROR_UNSUPPORTED, "backend does not report borrowed context data");
+  return ctx->HasBorrowedDataOfType(ctx, mem_type, has_borrowed_data_of_type);
+}
+
+static int ctx_check_valid(CeedQFunctionContext ctx, const char *what) {
+  bool valid;
+  CeedCall(CeedQFunctionContextHasValidData(ctx, &valid));
+  CeedCheck(valid, ctx->ceed, CEED_ERROR_BACKEND, "%s: the context holds no data yet (set it with CeedQFunctionContextSetData)", what);
+  return CEED_ERROR_SUCCESS;
+}
+
 int CeedQFunctionContextSetData(CeedQFunctionContext ctx, CeedMemType mem_type, CeedCopyMode copy_mode, size_t size, void *data) {
-  CeedCheck(mem_type == CEED_MEM_HOST, ctx->ceed, CEED_ERROR_UNSUPPORTED, "context data must be host memory");
-  free(ctx->owned);
-  ctx->owned = NULL;
-  ctx->size  = size;
-  switch (copy_mode) {
-    case CEED_COPY_VALUES:
-      CeedCall(CeedMallocArray(size ? size : 1, 1, &ctx->owned));
-      memcpy(ctx->owned, data, size);
-      ctx->host_data = ctx->owned;
-      break;
-    case CEED_OWN_POINTER:
-      ctx->owned = ctx->host_data = data;
-      break;
-    case CEED_USE_POINTER:
-      ctx->host_data = data;
-      break;
-  }
+  CeedCheck(ctx->SetData, ctx->ceed, CEED_ERROR_UNSUPPORTED, "backend does not store context data");
+  CeedCheck(ctx->state % 2 == 0, ctx->ceed, CEED_ERROR_ACCESS, "context data access already granted");
+  ctx->size = size;
+  CeedCall(ctx->SetData(ctx, mem_type, copy_mode, data));
   ctx->state += 2;
   return CEED_ERROR_SUCCESS;
 }
 
-int CeedQFunctionContextGetData(CeedQFunctionContext ctx, CeedMemType mem_type, void *data) {
-  CeedCheck(mem_type == CEED_MEM_HOST, ctx->ceed, CEED_ERROR_UNSUPPORTED, "context data is host memory");
+int CeedQFunctionContextTakeData(CeedQFunctionContext ctx, CeedMemType mem_type, void *data) {
+  bool  borrowed;
+  void *taken = NULL;
+  CeedCall(ctx_check_valid(ctx, "CeedQFunctionContextTakeData"));
+  CeedCheck(ctx->TakeData, ctx->ceed, CEED_ERROR_UNSUPPORTED, "backend does not hand context data back");
   CeedCheck(ctx->state % 2 == 0, ctx->ceed, CEED_ERROR_ACCESS, "context data access already granted");
-  *(void **)data = ctx->host_data;
+  CeedCall(CeedQFunctionContextHasBorrowedDataOfType(ctx, mem_type, &borrowed));
+  CeedCheck(borrowed, ctx->ceed, CEED_ERROR_BACKEND, "CeedQFunctionContextTakeData: no data of this memory type was lent with CEED_USE_POINTER");
+  CeedCall(ctx->TakeData(ctx, mem_type, &taken));
+  if (data) *(void **)data = taken;
+  return CEED_ERROR_SUCCESS;
+}
+
+int CeedQFunctionContextGetData(CeedQFunctionContext ctx, CeedMemType mem_type, void *data) {
+  CeedCheck(ctx->GetData, ctx->ceed, CEED_ERROR_UNSUPPORTED, "backend does not hand out context data");
+  CeedCheck(ctx->state % 2 == 0, ctx->ceed, CEED_ERROR_ACCESS, "context data access already granted");
+  CeedCheck(ctx->num_readers == 0, ctx->ceed, CEED_ERROR_ACCESS, "context data is being read");
+  CeedCall(ctx_check_valid(ctx, "CeedQFunctionContextGetData"));
+  CeedCall(ctx->GetData(ctx, mem_type, data));
   ctx->state++;
   return CEED_ERROR_SUCCESS;
 }
 
 int CeedQFunctionContextGetDataRead(CeedQFunctionContext ctx, CeedMemType mem_type, void *data) {
-  CeedCheck(mem_type == CEED_MEM_HOST, ctx->ceed, CEED_ERROR_UNSUPPORTED, "context data is host memory");
-  *(void **)data = ctx->host_data;
+  CeedCheck(ctx->GetDataRead, ctx->ceed, CEED_ERROR_UNSUPPORTED, "backend does not hand out context data");
+  CeedCheck(ctx->state % 2 == 0, ctx->ceed, CEED_ERROR_ACCESS, "context data access already granted");
+  CeedCall(ctx_check_valid(ctx, "CeedQFunctionContextGetDataRead"));
+  CeedCall(ctx->GetDataRead(ctx, mem_type, data));
+  ctx->num_readers++;
   return CEED_ERROR_SUCCESS;
 }
 
 int CeedQFunctionContextRestoreData(CeedQFunctionContext ctx, void *data) {
   CeedCheck(ctx->state % 2 == 1, ctx->ceed, CEED_ERROR_ACCESS, "context data access was not granted");
+  if (ctx->RestoreData) CeedCall(ctx->RestoreData(ctx));
   *(void **)data = NULL;
   ctx->state++;
   return CEED_ERROR_SUCCESS;
 }
 
 int CeedQFunctionContextRestoreDataRead(CeedQFunctionContext ctx, void *data) {
-  (void)ctx;
+  CeedCheck(ctx->num_readers > 0, ctx->ceed, CEED_ERROR_ACCESS, "context read access was not granted");
+  ctx->num_readers--;
+  if (ctx->num_readers == 0 && ctx->RestoreDataRead) CeedCall(ctx->RestoreDataRead(ctx));
   *(void **)data = NULL;
   return CEED_ERROR_SUCCESS;
 }
@@ -110,7 +138,6 @@ int CeedQFunctionContextDestroy(CeedQFunctionContext *ctx) {
     return CEED_ERROR_SUCCESS;
   }
   if ((*ctx)->Destroy) CeedCall((*ctx)->Destroy(*ctx));
-  free((*ctx)->owned);
   CeedCall(CeedDestroy(&(*ctx)->ceed));
   free(*ctx);
   *ctx = NULL;

@@ -1,8 +1,9 @@
 /*
  * CeedElemRestriction interface: creation (standard offsets or strides),
  * size checks and dispatch, following interface/ceed-elemrestriction.c
- * (create :657-680, strided :824, apply :1314-1336).  The core keeps a host
- * copy of the offsets (CeedElemRestrictionGetOffsets on host memory).
+ * (create :657-680, strided :824, apply :1314-1336).  The backend keeps the
+ * offsets (the caller's array under its copy mode) and hands them out through
+ * its GetOffsets slot (:269-299).
  */
 #include <stdlib.h>
 #include <string.h>
@@ -36,15 +37,12 @@ int CeedElemRestrictionCreate(Ceed ceed, CeedInt num_elem, CeedInt elem_size, Ce
   r->type               = CEED_RESTRICTION_STANDARD;
   r->comp_stride        = comp_stride;
   const size_t n        = (size_t)num_elem * elem_size;
-  CeedCall(CeedMalloc(n, &r->offsets));
-  memcpy(r->offsets, offsets, sizeof(CeedInt) * n);
   for (size_t i = 0; i < n; i++) {
-    const CeedSize max_l = (CeedSize)r->offsets[i] + (CeedSize)(num_comp - 1) * comp_stride;
-    CeedCheck(r->offsets[i] >= 0 && max_l < l_size, ceed, CEED_ERROR_DIMENSION, "restriction offset %d out of range for L-vector of size %td",
-              r->offsets[i], l_size);
+    const CeedSize max_l = (CeedSize)offsets[i] + (CeedSize)(num_comp - 1) * comp_stride;
+    CeedCheck(offsets[i] >= 0 && max_l < l_size, ceed, CEED_ERROR_DIMENSION, "restriction offset %d out of range for L-vector of size %td",
+              offsets[i], l_size);
   }
-  if (copy_mode == CEED_OWN_POINTER) free((void *)offsets);
-  CeedCall(ceed->ElemRestrictionCreate(mem_type, CEED_COPY_VALUES, r->offsets, NULL, NULL, r));
+  CeedCall(ceed->ElemRestrictionCreate(mem_type, copy_mode, offsets, NULL, NULL, r));
   return CEED_ERROR_SUCCESS;
 }
 
@@ -101,16 +99,15 @@ int CeedElemRestrictionHasBackendStrides(CeedElemRestriction rstr, bool *has_bac
 
 int CeedElemRestrictionGetOffsets(CeedElemRestriction rstr, CeedMemType mem_type, const CeedInt **offsets) {
   CeedCheck(rstr->type == CEED_RESTRICTION_STANDARD, rstr->ceed, CEED_ERROR_MINOR, "restriction has no offsets");
-  if (mem_type == CEED_MEM_HOST) {
-    *offsets = rstr->offsets;
-    return CEED_ERROR_SUCCESS;
-  }
-  CeedCheck(rstr->GetOffsets, rstr->ceed, CEED_ERROR_UNSUPPORTED, "backend does not provide device offsets");
-  return rstr->GetOffsets(rstr, mem_type, offsets);
+  CeedCheck(rstr->GetOffsets, rstr->ceed, CEED_ERROR_UNSUPPORTED, "backend does not hand out restriction offsets");
+  CeedCall(rstr->GetOffsets(rstr, mem_type, offsets));
+  rstr->num_readers++;
+  return CEED_ERROR_SUCCESS;
 }
 
 int CeedElemRestrictionRestoreOffsets(CeedElemRestriction rstr, const CeedInt **offsets) {
-  (void)rstr;
+  CeedCheck(rstr->num_readers > 0, rstr->ceed, CEED_ERROR_ACCESS, "no offsets array of this restriction is out");
+  rstr->num_readers--;
   *offsets = NULL;
   return CEED_ERROR_SUCCESS;
 }
@@ -199,7 +196,6 @@ int CeedElemRestrictionDestroy(CeedElemRestriction *rstr) {
     return CEED_ERROR_SUCCESS;
   }
   if ((*rstr)->Destroy) CeedCall((*rstr)->Destroy(*rstr));
-  free((*rstr)->offsets);
   CeedCall(CeedDestroy(&(*rstr)->ceed));
   free(*rstr);
   *rstr = NULL;

@@ -52,6 +52,28 @@ int CeedVectorGetLength(CeedVector vec, CeedSize *length) {
   return CEED_ERROR_SUCCESS;
 }
 
+int CeedVectorHasValidArray(CeedVector vec, bool *has_valid_array) {
+  CeedCheck(vec->HasValidArray, vec->ceed, CEED_ERROR_UNSUPPORTED, "backend does not report whether a CeedVector holds data");
+  if (vec->length == 0) {
+    *has_valid_array = true;
+    return CEED_ERROR_SUCCESS;
+  }
+  return vec->HasValidArray(vec, has_valid_array);
+}
+
+int CeedVectorHasBorrowedArrayOfType(CeedVector vec, CeedMemType mem_type, bool *has_borrowed_array_of_type) {
+  CeedCheck(vec->HasBorrowedArrayOfType, vec->ceed, CEED_ERROR_UNSUPPORTED, "backend does not report borrowed CeedVector arrays");
+  return vec->HasBorrowedArrayOfType(vec, mem_type, has_borrowed_array_of_type);
+}
+
+/* reads need data: set by CeedVectorSetArray, CeedVectorSetValue or a write access */
+static int check_valid(CeedVector vec, const char *what) {
+  bool valid;
+  CeedCall(CeedVectorHasValidArray(vec, &valid));
+  CeedCheck(valid, vec->ceed, CEED_ERROR_BACKEND, "%s: the CeedVector holds no data yet (set it with CeedVectorSetValue or CeedVectorSetArray)", what);
+  return CEED_ERROR_SUCCESS;
+}
+
 static int check_unlocked(CeedVector vec) {
   CeedCheck(vec->state % 2 == 0, vec->ceed, CEED_ERROR_ACCESS, "CeedVector access lock in use: a write access is out");
   CeedCheck(vec->num_readers == 0, vec->ceed, CEED_ERROR_ACCESS, "CeedVector access lock in use: a read access is out");
@@ -90,7 +112,13 @@ int CeedVectorTakeArray(CeedVector vec, CeedMemType mem_type, CeedScalar **array
   CeedCall(check_unlocked(vec));
   CeedCheck(vec->TakeArray, vec->ceed, CEED_ERROR_UNSUPPORTED, "backend does not support VectorTakeArray");
   *array = NULL;
-  if (vec->length > 0) CeedCall(vec->TakeArray(vec, mem_type, array));
+  if (vec->length > 0) {
+    bool borrowed;
+    CeedCall(CeedVectorHasBorrowedArrayOfType(vec, mem_type, &borrowed));
+    CeedCheck(borrowed, vec->ceed, CEED_ERROR_BACKEND, "CeedVectorTakeArray: no array of this memory type was lent with CEED_USE_POINTER");
+    CeedCall(check_valid(vec, "CeedVectorTakeArray"));
+    CeedCall(vec->TakeArray(vec, mem_type, array));
+  }
   return CEED_ERROR_SUCCESS;
 }
 
@@ -98,7 +126,10 @@ int CeedVectorGetArray(CeedVector vec, CeedMemType mem_type, CeedScalar **array)
   CeedCheck(vec->GetArray, vec->ceed, CEED_ERROR_UNSUPPORTED, "backend does not support VectorGetArray");
   CeedCall(check_unlocked(vec));
   *array = NULL;
-  if (vec->length > 0) CeedCall(vec->GetArray(vec, mem_type, array));
+  if (vec->length > 0) {
+    CeedCall(check_valid(vec, "CeedVectorGetArray"));
+    CeedCall(vec->GetArray(vec, mem_type, array));
+  }
   vec->state++;
   return CEED_ERROR_SUCCESS;
 }
@@ -107,7 +138,10 @@ int CeedVectorGetArrayRead(CeedVector vec, CeedMemType mem_type, const CeedScala
   CeedCheck(vec->GetArrayRead, vec->ceed, CEED_ERROR_UNSUPPORTED, "backend does not support VectorGetArrayRead");
   CeedCheck(vec->state % 2 == 0, vec->ceed, CEED_ERROR_ACCESS, "cannot grant read access while a write access is out");
   *array = NULL;
-  if (vec->length > 0) CeedCall(vec->GetArrayRead(vec, mem_type, array));
+  if (vec->length > 0) {
+    CeedCall(check_valid(vec, "CeedVectorGetArrayRead"));
+    CeedCall(vec->GetArrayRead(vec, mem_type, array));
+  }
   vec->num_readers++;
   return CEED_ERROR_SUCCESS;
 }
@@ -154,6 +188,7 @@ int CeedVectorNorm(CeedVector vec, CeedNormType type, CeedScalar *norm) {
   CeedCheck(vec->state % 2 == 0, vec->ceed, CEED_ERROR_ACCESS, "cannot compute a norm while a write access is out");
   *norm = 0.;
   if (vec->length == 0) return CEED_ERROR_SUCCESS;
+  CeedCall(check_valid(vec, "CeedVectorNorm"));
   if (vec->Norm) return vec->Norm(vec, type, norm);
   const CeedScalar *a;
   CeedCall(CeedVectorGetArrayRead(vec, CEED_MEM_HOST, &a));
@@ -170,7 +205,10 @@ int CeedVectorNorm(CeedVector vec, CeedNormType type, CeedScalar *norm) {
 int CeedVectorScale(CeedVector x, CeedScalar alpha) {
   CeedCall(check_unlocked(x));
   CeedCheck(x->Scale, x->ceed, CEED_ERROR_UNSUPPORTED, "backend does not support VectorScale");
-  if (x->length > 0) CeedCall(x->Scale(x, alpha));
+  if (x->length > 0) {
+    CeedCall(check_valid(x, "CeedVectorScale"));
+    CeedCall(x->Scale(x, alpha));
+  }
   x->state += 2;
   return CEED_ERROR_SUCCESS;
 }
@@ -180,7 +218,11 @@ int CeedVectorAXPY(CeedVector y, CeedScalar alpha, CeedVector x) {
   CeedCheck(x != y, y->ceed, CEED_ERROR_UNSUPPORTED, "cannot use same vector for x and y in CeedVectorAXPY");
   CeedCall(check_unlocked(y));
   CeedCheck(y->AXPY, y->ceed, CEED_ERROR_UNSUPPORTED, "backend does not support VectorAXPY");
-  if (y->length > 0) CeedCall(y->AXPY(y, alpha, x));
+  if (y->length > 0) {
+    CeedCall(check_valid(x, "CeedVectorAXPY (x)"));
+    CeedCall(check_valid(y, "CeedVectorAXPY (y)"));
+    CeedCall(y->AXPY(y, alpha, x));
+  }
   y->state += 2;
   return CEED_ERROR_SUCCESS;
 }
@@ -190,7 +232,11 @@ int CeedVectorAXPBY(CeedVector y, CeedScalar alpha, CeedScalar beta, CeedVector 
   CeedCheck(x != y, y->ceed, CEED_ERROR_UNSUPPORTED, "cannot use same vector for x and y in CeedVectorAXPBY");
   CeedCall(check_unlocked(y));
   CeedCheck(y->AXPBY, y->ceed, CEED_ERROR_UNSUPPORTED, "backend does not support VectorAXPBY");
-  if (y->length > 0) CeedCall(y->AXPBY(y, alpha, beta, x));
+  if (y->length > 0) {
+    CeedCall(check_valid(x, "CeedVectorAXPBY (x)"));
+    CeedCall(check_valid(y, "CeedVectorAXPBY (y)"));
+    CeedCall(y->AXPBY(y, alpha, beta, x));
+  }
   y->state += 2;
   return CEED_ERROR_SUCCESS;
 }
@@ -199,7 +245,11 @@ int CeedVectorPointwiseMult(CeedVector w, CeedVector x, CeedVector y) {
   CeedCheck(w->length == x->length && w->length == y->length, w->ceed, CEED_ERROR_UNSUPPORTED, "vector lengths differ");
   CeedCall(check_unlocked(w));
   CeedCheck(w->PointwiseMult, w->ceed, CEED_ERROR_UNSUPPORTED, "backend does not support VectorPointwiseMult");
-  if (w->length > 0) CeedCall(w->PointwiseMult(w, x, y));
+  if (w->length > 0) {
+    CeedCall(check_valid(x, "CeedVectorPointwiseMult (x)"));
+    CeedCall(check_valid(y, "CeedVectorPointwiseMult (y)"));
+    CeedCall(w->PointwiseMult(w, x, y));
+  }
   w->state += 2;
   return CEED_ERROR_SUCCESS;
 }
@@ -207,7 +257,10 @@ int CeedVectorPointwiseMult(CeedVector w, CeedVector x, CeedVector y) {
 int CeedVectorReciprocal(CeedVector vec) {
   CeedCall(check_unlocked(vec));
   CeedCheck(vec->Reciprocal, vec->ceed, CEED_ERROR_UNSUPPORTED, "backend does not support VectorReciprocal");
-  if (vec->length > 0) CeedCall(vec->Reciprocal(vec));
+  if (vec->length > 0) {
+    CeedCall(check_valid(vec, "CeedVectorReciprocal"));
+    CeedCall(vec->Reciprocal(vec));
+  }
   vec->state += 2;
   return CEED_ERROR_SUCCESS;
 }

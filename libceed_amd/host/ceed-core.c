@@ -330,6 +330,8 @@ static const SlotEntry slots[] = {
     SLOT(Ceed, Ceed_private, QFunctionCreate),
     SLOT(Ceed, Ceed_private, QFunctionContextCreate),
     SLOT(Ceed, Ceed_private, OperatorCreate),
+    SLOT(Vector, CeedVector_private, HasValidArray),
+    SLOT(Vector, CeedVector_private, HasBorrowedArrayOfType),
     SLOT(Vector, CeedVector_private, SetArray),
     SLOT(Vector, CeedVector_private, TakeArray),
     SLOT(Vector, CeedVector_private, SyncArray),
@@ -354,6 +356,14 @@ static const SlotEntry slots[] = {
     SLOT(Basis, CeedBasis_private, Destroy),
     SLOT(QFunction, CeedQFunction_private, Apply),
     SLOT(QFunction, CeedQFunction_private, Destroy),
+    SLOT(QFunctionContext, CeedQFunctionContext_private, HasValidData),
+    SLOT(QFunctionContext, CeedQFunctionContext_private, HasBorrowedDataOfType),
+    SLOT(QFunctionContext, CeedQFunctionContext_private, SetData),
+    SLOT(QFunctionContext, CeedQFunctionContext_private, TakeData),
+    SLOT(QFunctionContext, CeedQFunctionContext_private, GetData),
+    SLOT(QFunctionContext, CeedQFunctionContext_private, GetDataRead),
+    SLOT(QFunctionContext, CeedQFunctionContext_private, RestoreData),
+    SLOT(QFunctionContext, CeedQFunctionContext_private, RestoreDataRead),
     SLOT(QFunctionContext, CeedQFunctionContext_private, Destroy),
     SLOT(Operator, CeedOperator_private, Apply),
     SLOT(Operator, CeedOperator_private, ApplyAdd),

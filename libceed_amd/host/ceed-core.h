@@ -37,6 +37,8 @@ struct CeedVector_private {
   uint64_t state;      /* odd while a write access is out */
   int      num_readers;
   void    *data;
+  int (*HasValidArray)(CeedVector, bool *);
+  int (*HasBorrowedArrayOfType)(CeedVector, CeedMemType, bool *);
   int (*SetArray)(CeedVector, CeedMemType, CeedCopyMode, CeedScalar *);
   int (*TakeArray)(CeedVector, CeedMemType, CeedScalar **);
   int (*SyncArray)(CeedVector, CeedMemType);
@@ -64,7 +66,7 @@ struct CeedElemRestriction_private {
   CeedInt             strides[3];
   bool                backend_strides;
   CeedInt             e_layout[3];
-  CeedInt            *offsets; /* host copy (standard restrictions) */
+  uint64_t            num_readers; /* offsets arrays handed out and not yet restored */
   void               *data;
   int (*Apply)(CeedElemRestriction, CeedTransposeMode, CeedVector, CeedVector, CeedRequest *);
   int (*GetOffsets)(CeedElemRestriction, CeedMemType, const CeedInt **);
@@ -92,9 +94,16 @@ struct CeedQFunctionContext_private {
   Ceed     ceed;
   int      ref_count;
   size_t   size;
-  void    *host_data, *owned;
-  uint64_t state;
-  void    *backend_data;
+  uint64_t state, num_readers;
+  void    *backend_data; /* the backend stores the data itself */
+  int (*HasValidData)(CeedQFunctionContext, bool *);
+  int (*HasBorrowedDataOfType)(CeedQFunctionContext, CeedMemType, bool *);
+  int (*SetData)(CeedQFunctionContext, CeedMemType, CeedCopyMode, void *);
+  int (*TakeData)(CeedQFunctionContext, CeedMemType, void *);
+  int (*GetData)(CeedQFunctionContext, CeedMemType, void *);
+  int (*GetDataRead)(CeedQFunctionContext, CeedMemType, void *);
+  int (*RestoreData)(CeedQFunctionContext);
+  int (*RestoreDataRead)(CeedQFunctionContext);
   int (*Destroy)(CeedQFunctionContext);
 };
 

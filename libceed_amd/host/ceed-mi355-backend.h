@@ -12,6 +12,7 @@
 typedef struct {
   CeedMi355Restriction handle;
   CeedRestrictionType  type;
+  CeedInt             *offsets, *offsets_owned; /* host offsets (standard restrictions) */
 } RestrictionData;
 
 typedef struct {
@@ -23,9 +24,11 @@ typedef struct {
 } QFunctionData;
 
 typedef struct {
-  void    *device; /* device copy of the context bytes */
-  size_t   size;
-  uint64_t state;  /* context state the copy was taken at */
+  void    *data, *owned;   /* the context bytes (host memory), and the allocation the backend frees */
+  void    *device;         /* device copy of the bytes */
+  size_t   device_size;
+  uint64_t device_state;   /* context state the copy was taken at */
+  bool     has_device_copy;
 } ContextData;
 
 /* Turn a kernel-library status into a CeedError carrying its message. */

@@ -184,6 +184,23 @@ static int VectorRestoreArray(CeedVector vec) {
   return CEED_ERROR_SUCCESS;
 }
 
+/* valid data on either side (interface/ceed-vector.c:89 asks before every read) */
+static int VectorHasValidArray(CeedVector vec, bool *has_valid_array) {
+  VectorData *d;
+  CeedCallBackend(CeedVectorGetData(vec, &d));
+  *has_valid_array = d->h_valid || d->d_valid;
+  return CEED_ERROR_SUCCESS;
+}
+
+/* an array of the given memory type set with CEED_USE_POINTER (CeedVectorTakeArray's precondition) */
+static int VectorHasBorrowedArrayOfType(CeedVector vec, CeedMemType mt, bool *has_borrowed) {
+  VectorData *d;
+  CeedCallBackend(CeedVectorGetData(vec, &d));
+  if (mt == CEED_MEM_HOST) *has_borrowed = d->h_array && d->h_array != d->h_owned;
+  else *has_borrowed = d->d_array && d->d_array != d->d_owned;
+  return CEED_ERROR_SUCCESS;
+}
+
 int CeedMi355VectorDevice(CeedVector vec, bool write, CeedScalar **array) {
   VectorData *d;
   CeedCallBackend(CeedVectorGetData(vec, &d));
@@ -264,6 +281,8 @@ static int VectorCreate(CeedSize n, CeedVector vec) {
   (void)n;
   CeedCallBackend(CeedVectorGetCeed(vec, &ceed));
   CeedCallBackend(CeedVectorSetData(vec, d));
+  CeedCallBackend(CeedSetBackendFunction(ceed, "Vector", vec, "HasValidArray", VectorHasValidArray));
+  CeedCallBackend(CeedSetBackendFunction(ceed, "Vector", vec, "HasBorrowedArrayOfType", VectorHasBorrowedArrayOfType));
   CeedCallBackend(CeedSetBackendFunction(ceed, "Vector", vec, "SetArray", VectorSetArray));
   CeedCallBackend(CeedSetBackendFunction(ceed, "Vector", vec, "TakeArray", VectorTakeArray));
   CeedCallBackend(CeedSetBackendFunction(ceed, "Vector", vec, "SyncArray", VectorSyncArray));
@@ -306,10 +325,25 @@ static int RestrictionApply(CeedElemRestriction rstr, CeedTransposeMode t_mode, 
   return CEED_ERROR_SUCCESS;
 }
 
+/* the offsets as created, on the host (the device copy lives inside the kernel
+ * library's restriction and is not handed out) */
+static int RestrictionGetOffsets(CeedElemRestriction rstr, CeedMemType mt, const CeedInt **offsets) {
+  RestrictionData *d;
+  Ceed             ceed;
+  CeedCallBackend(CeedElemRestrictionGetData(rstr, &d));
+  CeedCallBackend(CeedElemRestrictionGetCeed(rstr, &ceed));
+  CeedCheck(d->offsets, ceed, CEED_ERROR_BACKEND, "a strided restriction has no offsets");
+  CeedCheck(mt == CEED_MEM_HOST, ceed, CEED_ERROR_UNSUPPORTED, "/gpu/hip/mi355 provides restriction offsets in host memory only");
+  *offsets = d->offsets;
+  CeedCallBackend(CeedDestroy(&ceed));
+  return CEED_ERROR_SUCCESS;
+}
+
 static int RestrictionDestroy(CeedElemRestriction rstr) {
   RestrictionData *d;
   CeedCallBackend(CeedElemRestrictionGetData(rstr, &d));
   CeedMi355RestrictionDestroy(&d->handle);
+  free(d->offsets_owned);
   free(d);
   return CEED_ERROR_SUCCESS;
 }
@@ -320,7 +354,6 @@ static int ElemRestrictionCreate(CeedMemType mt, CeedCopyMode cm, const CeedInt 
   CeedInt             num_elem, elem_size, num_comp, comp_stride = 1;
   CeedSize            l_size;
   CeedRestrictionType type;
-  (void)cm;
   CeedCallBackend(CeedElemRestrictionGetCeed(rstr, &ceed));
   CeedCheck(mt == CEED_MEM_HOST, ceed, CEED_ERROR_BACKEND, "offsets must be host memory");
   CeedCheck(!orients && !curl_orients, ceed, CEED_ERROR_UNSUPPORTED, "oriented restrictions are not supported by /gpu/hip/mi355");
@@ -333,12 +366,32 @@ static int ElemRestrictionCreate(CeedMemType mt, CeedCopyMode cm, const CeedInt 
   d->type            = type;
   if (type == CEED_RESTRICTION_STRIDED) {
     CeedInt strides[3];
-    CeedCallBackend(CeedElemRestrictionGetStrides(rstr, strides));
+    bool    backend_strides;
+    CeedCallBackend(CeedElemRestrictionHasBackendStrides(rstr, &backend_strides));
+    if (backend_strides) { /* CEED_STRIDES_BACKEND: this backend's E-vector layout, [elem][comp][node] */
+      strides[0] = 1;
+      strides[1] = elem_size;
+      strides[2] = elem_size * num_comp;
+    } else {
+      CeedCallBackend(CeedElemRestrictionGetStrides(rstr, strides));
+    }
     CeedCallBackend(CeedMi355Check(ceed, CeedMi355RestrictionCreateStrided(num_elem, elem_size, num_comp, l_size, strides, &d->handle)));
   } else {
     CeedCheck(type == CEED_RESTRICTION_STANDARD, ceed, CEED_ERROR_UNSUPPORTED, "restriction type not supported by /gpu/hip/mi355");
     CeedCallBackend(CeedElemRestrictionGetCompStride(rstr, &comp_stride));
     CeedCallBackend(CeedMi355Check(ceed, CeedMi355RestrictionCreateOffset(num_elem, elem_size, num_comp, comp_stride, l_size, offsets, &d->handle)));
+    /* host offsets for CeedElemRestrictionGetOffsets: the caller's array when
+     * it is handed over (OWN) or lent (USE), else a copy */
+    const size_t n = (size_t)num_elem * elem_size;
+    if (cm == CEED_COPY_VALUES) {
+      d->offsets_owned = malloc(sizeof(CeedInt) * (n ? n : 1));
+      CeedCheck(d->offsets_owned, ceed, CEED_ERROR_MAJOR, "host allocation failed");
+      if (n) memcpy(d->offsets_owned, offsets, sizeof(CeedInt) * n);
+      d->offsets = d->offsets_owned;
+    } else {
+      d->offsets       = (CeedInt *)offsets;
+      d->offsets_owned = cm == CEED_OWN_POINTER ? (CeedInt *)offsets : NULL;
+    }
   }
   {
     CeedInt layout[3] = {1, elem_size, elem_size * num_comp}; /* E-vectors are [elem][comp][node] */
@@ -346,6 +399,7 @@ static int ElemRestrictionCreate(CeedMemType mt, CeedCopyMode cm, const CeedInt 
   }
   CeedCallBackend(CeedElemRestrictionSetData(rstr, d));
   CeedCallBackend(CeedSetBackendFunction(ceed, "ElemRestriction", rstr, "Apply", RestrictionApply));
+  CeedCallBackend(CeedSetBackendFunction(ceed, "ElemRestriction", rstr, "GetOffsets", RestrictionGetOffsets));
   CeedCallBackend(CeedSetBackendFunction(ceed, "ElemRestriction", rstr, "Destroy", RestrictionDestroy));
   CeedCallBackend(CeedDestroy(&ceed));
   return CEED_ERROR_SUCCESS;
@@ -439,24 +493,20 @@ int CeedMi355QFunctionContextDevice(CeedQFunction qf, void **ctx_device) {
   size_t       size;
   CeedCallBackend(CeedQFunctionContextGetCeed(ctx, &ceed));
   CeedCallBackend(CeedQFunctionContextGetBackendData(ctx, &d));
-  if (!d) {
-    d = calloc(1, sizeof(ContextData));
-    d->state = UINT64_MAX;
-    CeedCallBackend(CeedQFunctionContextSetBackendData(ctx, d));
-  }
   CeedCallBackend(CeedQFunctionContextGetState(ctx, &state));
   CeedCallBackend(CeedQFunctionContextGetContextSize(ctx, &size));
-  if (state != d->state && size > 0) {
+  if ((!d->has_device_copy || state != d->device_state) && size > 0 && d->data) {
     void *host;
-    if (!d->device || d->size < size) {
+    if (!d->device || d->device_size < size) {
       CeedMi355Free(d->device);
       CeedCallBackend(dev_alloc(ceed, size, &d->device));
-      d->size = size;
+      d->device_size = size;
     }
     CeedCallBackend(CeedQFunctionContextGetDataRead(ctx, CEED_MEM_HOST, &host));
     CeedCallBackend(copy_sync(ceed, d->device, host, size));
     CeedCallBackend(CeedQFunctionContextRestoreDataRead(ctx, &host));
-    d->state = state;
+    d->device_state    = state;
+    d->has_device_copy = true;
   }
   *ctx_device = d->device;
   CeedCallBackend(CeedDestroy(&ceed));
@@ -503,19 +553,103 @@ static int QFunctionCreate(CeedQFunction qf) {
   return CEED_ERROR_SUCCESS;
 }
 
+/* The context bytes live in host memory (QFunctions read them through a
+ * device copy refreshed on state change, CeedMi355QFunctionContextDevice);
+ * the slots follow the backend contract of interface/ceed-qfunctioncontext.c. */
+static int context_data(CeedQFunctionContext ctx, CeedMemType mt, ContextData **d) {
+  CeedCallBackend(CeedQFunctionContextGetBackendData(ctx, d));
+  if (mt != CEED_MEM_HOST) {
+    Ceed ceed;
+    CeedCallBackend(CeedQFunctionContextGetCeed(ctx, &ceed));
+    return CeedError(ceed, CEED_ERROR_UNSUPPORTED, "/gpu/hip/mi355 keeps QFunction context data in host memory");
+  }
+  return CEED_ERROR_SUCCESS;
+}
+
+static int ContextHasValidData(CeedQFunctionContext ctx, bool *has_valid_data) {
+  ContextData *d;
+  CeedCallBackend(CeedQFunctionContextGetBackendData(ctx, &d));
+  *has_valid_data = d->data != NULL;
+  return CEED_ERROR_SUCCESS;
+}
+
+static int ContextHasBorrowedDataOfType(CeedQFunctionContext ctx, CeedMemType mt, bool *has_borrowed) {
+  ContextData *d;
+  CeedCallBackend(CeedQFunctionContextGetBackendData(ctx, &d));
+  *has_borrowed = mt == CEED_MEM_HOST && d->data && d->data != d->owned;
+  return CEED_ERROR_SUCCESS;
+}
+
+static int ContextSetData(CeedQFunctionContext ctx, CeedMemType mt, CeedCopyMode cm, void *data) {
+  ContextData *d;
+  size_t       size;
+  CeedCallBackend(context_data(ctx, mt, &d));
+  CeedCallBackend(CeedQFunctionContextGetContextSize(ctx, &size));
+  free(d->owned);
+  d->owned = NULL;
+  if (cm == CEED_COPY_VALUES) {
+    d->owned = malloc(size ? size : 1);
+    if (!d->owned) {
+      Ceed ceed;
+      CeedCallBackend(CeedQFunctionContextGetCeed(ctx, &ceed));
+      return CeedError(ceed, CEED_ERROR_MAJOR, "host allocation failed");
+    }
+    if (size) memcpy(d->owned, data, size);
+    d->data = d->owned;
+  } else {
+    d->data  = data;
+    d->owned = cm == CEED_OWN_POINTER ? data : NULL;
+  }
+  d->has_device_copy = false;
+  return CEED_ERROR_SUCCESS;
+}
+
+static int ContextTakeData(CeedQFunctionContext ctx, CeedMemType mt, void *data) {
+  ContextData *d;
+  CeedCallBackend(context_data(ctx, mt, &d));
+  *(void **)data     = d->data;
+  d->data            = NULL;
+  d->has_device_copy = false;
+  return CEED_ERROR_SUCCESS;
+}
+
+static int ContextGetData(CeedQFunctionContext ctx, CeedMemType mt, void *data) {
+  ContextData *d;
+  CeedCallBackend(context_data(ctx, mt, &d));
+  *(void **)data = d->data;
+  return CEED_ERROR_SUCCESS;
+}
+
+static int ContextRestoreData(CeedQFunctionContext ctx) {
+  (void)ctx;
+  return CEED_ERROR_SUCCESS;
+}
+
 static int QFunctionContextDestroy(CeedQFunctionContext ctx) {
   ContextData *d;
   CeedCallBackend(CeedQFunctionContextGetBackendData(ctx, &d));
   if (d) {
     CeedMi355Free(d->device);
+    free(d->owned);
     free(d);
   }
   return CEED_ERROR_SUCCESS;
 }
 
 static int QFunctionContextCreate(CeedQFunctionContext ctx) {
-  Ceed ceed;
+  Ceed         ceed;
+  ContextData *d = calloc(1, sizeof(ContextData));
   CeedCallBackend(CeedQFunctionContextGetCeed(ctx, &ceed));
+  CeedCheck(d, ceed, CEED_ERROR_MAJOR, "host allocation failed");
+  CeedCallBackend(CeedQFunctionContextSetBackendData(ctx, d));
+  CeedCallBackend(CeedSetBackendFunction(ceed, "QFunctionContext", ctx, "HasValidData", ContextHasValidData));
+  CeedCallBackend(CeedSetBackendFunction(ceed, "QFunctionContext", ctx, "HasBorrowedDataOfType", ContextHasBorrowedDataOfType));
+  CeedCallBackend(CeedSetBackendFunction(ceed, "QFunctionContext", ctx, "SetData", ContextSetData));
+  CeedCallBackend(CeedSetBackendFunction(ceed, "QFunctionContext", ctx, "TakeData", ContextTakeData));
+  CeedCallBackend(CeedSetBackendFunction(ceed, "QFunctionContext", ctx, "GetData", ContextGetData));
+  CeedCallBackend(CeedSetBackendFunction(ceed, "QFunctionContext", ctx, "GetDataRead", ContextGetData));
+  CeedCallBackend(CeedSetBackendFunction(ceed, "QFunctionContext", ctx, "RestoreData", ContextRestoreData));
+  CeedCallBackend(CeedSetBackendFunction(ceed, "QFunctionContext", ctx, "RestoreDataRead", ContextRestoreData));
   CeedCallBackend(CeedSetBackendFunction(ceed, "QFunctionContext", ctx, "Destroy", QFunctionContextDestroy));
   CeedCallBackend(CeedDestroy(&ceed));
   return CEED_ERROR_SUCCESS;
@@ -547,7 +681,21 @@ int CeedInit_Mi355(const char *resource, Ceed ceed) {
   int *data = calloc(1, sizeof(int));
   *data     = device_id;
   CeedCallBackend(CeedSetData(ceed, data));
-  CeedCallBackend(CeedSetDeterministic(ceed, true)); /* every E^T sums in a fixed order, without atomics */
+  {
+    /* Every default E^T mode gives the same bits on every run: colours, the
+     * transpose map and bricks sum in a fixed order, and the multiplicity
+     * split's atomic adds are two-term sums, which do not depend on the order.
+     * CEED_MI355_SCATTER=atomic switches plans to order-dependent atomics. */
+    const char *scatter = getenv("CEED_MI355_SCATTER");
+    bool        deterministic = true;
+    if (scatter) {
+      CeedCheck(!strcmp(scatter, "color") || !strcmp(scatter, "tmap") || !strcmp(scatter, "split") || !strcmp(scatter, "atomic") ||
+                    !strcmp(scatter, "brick"),
+                ceed, CEED_ERROR_BACKEND, "CEED_MI355_SCATTER=%s: expected color, tmap, split, atomic or brick", scatter);
+      deterministic = strcmp(scatter, "atomic") != 0;
+    }
+    CeedCallBackend(CeedSetDeterministic(ceed, deterministic));
+  }
   CeedCallBackend(CeedSetBackendFunction(ceed, "Ceed", ceed, "GetPreferredMemType", GetPreferredMemType));
   CeedCallBackend(CeedSetBackendFunction(ceed, "Ceed", ceed, "Destroy", CeedDestroy_Mi355));
   CeedCallBackend(CeedSetBackendFunction(ceed, "Ceed", ceed, "VectorCreate", VectorCreate));

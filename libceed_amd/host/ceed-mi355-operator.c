@@ -157,16 +157,15 @@ static int try_fused(CeedOperator op, OperatorData *d) {
     CeedCallBackend(CeedElemRestrictionGetLVectorSize(ru, &d->l_size));
     CeedCallBackend(CeedElemRestrictionGetOffsets(ru, CEED_MEM_HOST, &offsets));
     /* shapes the fused kernels are instantiated for; anything else takes the generic path */
-    if (elem_size == P * P * P && P >= 2 && P <= 9 && (Q == P || Q == P + 1)) {
-      int ierr = CeedMi355OperatorCreateHex(kind, CEED_MI355_FP64, num_comp, num_elem, P, Q, interp, grad, offsets, comp_stride, d->l_size, 0,
-                                            &d->hex);
-      if (ierr == CEED_MI355_ERROR_UNSUPPORTED) ok = 0;
-      else {
-        CeedCallBackend(CeedMi355Check(ceed, ierr));
-        ok = 1;
-      }
-    }
+    int ierr = CEED_MI355_ERROR_UNSUPPORTED;
+    if (elem_size == P * P * P && P >= 2 && P <= 9 && (Q == P || Q == P + 1))
+      ierr = CeedMi355OperatorCreateHex(kind, CEED_MI355_FP64, num_comp, num_elem, P, Q, interp, grad, offsets, comp_stride, d->l_size, 0,
+                                        &d->hex);
     CeedCallBackend(CeedElemRestrictionRestoreOffsets(ru, &offsets));
+    if (ierr != CEED_MI355_ERROR_UNSUPPORTED) {
+      CeedCallBackend(CeedMi355Check(ceed, ierr));
+      ok = 1;
+    }
   }
 done:
   d->fused       = ok;
@@ -365,7 +364,7 @@ static int generic_setup(CeedOperator op, OperatorData *d) {
     CeedElemRestriction rstr;
     CeedBasis           basis;
     CeedEvalMode        mode;
-    CeedInt             per_point;
+    CeedInt             per_point = 0;
     CeedCallBackend(CeedOperatorFieldGetElemRestriction(is_in ? in[i] : out[i], &rstr));
     CeedCallBackend(CeedOperatorFieldGetBasis(is_in ? in[i] : out[i], &basis));
     CeedCallBackend(CeedQFunctionFieldGetEvalMode(is_in ? qf_in[i] : qf_out[i], &mode));

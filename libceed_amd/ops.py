@@ -25,7 +25,7 @@ FP64, FP32 = 0, 1
 # FP64, both on meshes of at least 10M E-vector entries; the transpose map otherwise), transpose offset map, element
 # colours, multiplicity split (atomic adds for entries of at most two contributions, the ordered map for the rest),
 # hardware atomics
-SCATTER_FLAGS = {"auto": 0, "tmap": 8, "color": 2, "atomic": 4, "split": 16}
+SCATTER_FLAGS = {"auto": 0, "tmap": 8, "color": 2, "atomic": 4, "split": 16, "brick": 32}
 GAUSS, GAUSS_LOBATTO = 0, 1
 
 
@@ -240,7 +240,7 @@ class HexOperator:
         return best.value, ms.value
 
     def scatter_mode(self) -> str:
-        """The plan's E^T mode: "tmap", "color", "split" or "atomic"."""
+        """The plan's E^T mode: "tmap", "color", "split", "atomic" or "brick"."""
         mode = C.c_int32()
         call("CeedMi355OperatorGetScatter", self._h, C.byref(mode))
         return {v: k for k, v in SCATTER_FLAGS.items() if k != "auto"}[mode.value]

@@ -34,7 +34,9 @@ def test_diagonal_vs_dense(cuda, kind, nc, p, Q):
     qd = oracle.hex_build_qdata(0 if kind == 0 else 1, m, Q)
     kappa = np.exp(np.random.default_rng(1).uniform(-0.5, 0.5, m.num_elem * Q ** 3)) if kind == 2 else None
     ref = oracle_diag(kind, nc, m, Q, qd, kappa)
-    for scatter in ("tmap", "color", "split", "atomic"):
+    for scatter in ("tmap", "color", "split", "atomic", "brick"):
+        if scatter == "brick" and kind == 2:  # no brick E^T for the kappa-weighted plan
+            continue
         op = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, scatter=scatter)
         if kind == 2:
             op.set_kappa(torch.from_numpy(kappa).cuda())

@@ -163,7 +163,7 @@ def test_operator_vs_oracle(cuda, kind, nc, n, p, Q):
     u = rng.uniform(-1, 1, nc * m.num_nodes)
     qd_ref = oracle.hex_build_qdata(kind, m, Q)
     v_ref = oracle.hex_apply(kind, nc, m.num_elem, p + 1, Q, m.offsets, m.num_nodes, qd_ref, u)
-    for det in ("tmap", "color", "atomic"):  # the three E^T modes of the fused plan
+    for det in ("tmap", "color", "atomic", "brick"):  # the E^T modes of the fused plan
         # geometric factors built on the GPU
         op = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, scatter=det)
         assert rel_l2(host(op.get_qdata()), qd_ref) < RTOL64
@@ -274,7 +274,7 @@ def test_mass_config_repeated_apply(cuda):
     qd = oracle.hex_build_qdata(0, m, 5)
     v_ref = oracle.hex_apply(0, 1, m.num_elem, 4, 5, m.offsets, m.num_nodes, qd, u)
     ud, v = dev(u), torch.zeros(m.num_nodes, dtype=torch.float64, device="cuda")
-    for scatter in ("tmap", "color"):  # the atomic-free E^T modes: bitwise repeatable
+    for scatter in ("tmap", "color", "brick"):  # the atomic-free E^T modes: bitwise repeatable
         op = ops.HexOperator.from_mesh(ops.QF_MASS, m, Q=5, scatter=scatter)
         op.apply(ud, v)
         first = v.clone()
@@ -358,7 +358,7 @@ def test_default_modes_full_size(cuda, p, mode):
     assert Ab.abs().max().item() < 1e-10
 
 
-@pytest.mark.parametrize("scatter", ["tmap", "color", "split", "atomic"])
+@pytest.mark.parametrize("scatter", ["tmap", "color", "split", "atomic", "brick"])
 @pytest.mark.parametrize("kind,nc,p", [(1, 1, 3), (0, 1, 4), (1, 3, 2)])
 def test_operator_unstructured_numbering(cuda, scatter, kind, nc, p):
     """Nodes renumbered by a random permutation into an L-vector with unused entries, elements in random order
@@ -395,7 +395,7 @@ def test_operator_unstructured_numbering(cuda, scatter, kind, nc, p):
     assert rel_l2(wh, w0 + v_ref) < RTOL64
 
 
-@pytest.mark.parametrize("scatter", ["tmap", "color", "split", "atomic"])
+@pytest.mark.parametrize("scatter", ["tmap", "color", "split", "atomic", "brick"])
 def test_operator_no_elements(cuda, scatter):
     """A plan with no elements: Apply gives zeros, ApplyAdd leaves v unchanged."""
     op = ops.HexOperator(ops.QF_DIFFUSION, np.zeros(0, dtype=np.int32), 0, 3, 4, 10, scatter=scatter)
