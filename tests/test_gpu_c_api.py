@@ -3,7 +3,8 @@ tests/c/ex-hex.c (the kind of program examples/ceed/ex1-volume.c and
 ex2-surface.c are: exact volume and energy of an affine hex mesh, and the
 fused gallery operator equals the staged user-QFunction operator to 1e-12)
 and tests/c/bp-apply.c (CeedOperatorApply throughput of the gallery Poisson
-operators, scalar and 3-component)."""
+operators, scalar and 3-component, on the perturbed trilinear mesh, timed by
+one synchronised loop; A 1 = 0 checked)."""
 
 import json
 import math
@@ -32,5 +33,6 @@ def test_bp_apply(cuda, p, n, nc):
     print(res.stdout, res.stderr)
     assert res.returncode == 0, res.stdout + res.stderr
     r = json.loads(res.stdout.strip().splitlines()[-1])
-    # the time is a difference of two host-timed runs: at these sizes it is noise and may come out negative
-    assert r["dofs"] == nc * (n * p + 1) ** 3 and math.isfinite(r["ms_per_apply"])
+    assert r["dofs"] == nc * (n * p + 1) ** 3
+    assert math.isfinite(r["ms_per_apply"]) and r["ms_per_apply"] > 0
+    assert r["ones_max_abs"] < 1e-11  # pure Neumann diffusion: A 1 = 0 on the perturbed trilinear mesh
