@@ -326,34 +326,75 @@ def run_batch(count: int = 256, rtol: float = 1e-8, seed: int = 0) -> dict:
     return out
 
 
-def cpu_baseline(p: int = 4, seconds: float = 10.0) -> dict | None:
-    """Reference /cpu/self/avx/blocked on this host's cores (one single-threaded process per core)."""
-    exe = ROOT / "oracle" / "_ref" / "ref_cpu_bench"
-    cores = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1))
-    n = 12  # 12^3 elements at p = 4: 117,649 DoFs per process
-    if exe.exists():
-        procs = [subprocess.Popen([str(exe), "/cpu/self/avx/blocked", str(p), str(n), str(seconds)], stdout=subprocess.PIPE, text=True)
-                 for _ in range(cores)]
-        rates, dofs = [], 0
-        for pr in procs:
-            out, _ = pr.communicate(timeout=seconds * 6 + 120)
-            if pr.returncode != 0:
-                return None
-            r = json.loads(out.strip().splitlines()[-1])
-            rates.append(r["dofs"] * r["applies"] / r["seconds"])
-            dofs = r["dofs"]
-        cpu_model = ""
+def physical_cores() -> tuple[int, int]:
+    """(physical cores, logical CPUs) this process may run on: distinct (package, core) pairs of the affinity set,
+    capped by OMP_NUM_THREADS where the host sets the job's CPU share there."""
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        t = Path(f"/sys/devices/system/cpu/cpu{c}/topology")
         try:
-            for line in open("/proc/cpuinfo"):
-                if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
-                    break
+            cores.add(((t / "physical_package_id").read_text().strip(), (t / "core_id").read_text().strip()))
         except OSError:
-            pass
-        return {"value": sum(rates) / 1e9, "unit": "GDoF/s", "cores": cores, "kind": "reference",
-                "sample": f"/cpu/self/avx/blocked (reference sources, -O3 -march=x86-64-v3), 3D Poisson p={p} Q={p + 2}, "
-                          f"{n}^3 elements ({dofs} DoFs) per process, {cores} processes x {seconds:.0f} s",
-                "cpu": cpu_model}
+            cores.add(("?", str(c)))
+    n = len(cores)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n), len(cpus)
+
+
+def cpu_model() -> tuple[str, bool]:
+    """Model name and whether the CPU has the x86-64-v4 (AVX-512 F/CD/BW/DQ/VL) feature set."""
+    model, flags = "", set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and not model:
+                model = line.split(":", 1)[1].strip()
+            elif line.startswith("flags") and not flags:
+                flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return model, {"avx512f", "avx512cd", "avx512bw", "avx512dq", "avx512vl"} <= flags
+
+
+def cpu_baseline(p: int = 4, n: int = 50, seconds: float = 10.0) -> dict | None:
+    """Reference /cpu/self/avx/blocked (built from its sources, oracle/_ref) on this host: one single-threaded process
+    per physical core, each on an equal slab of the GPU workload's n^3-element mesh (n x n x n/cores elements)."""
+    exes = {"x86-64-v3": ROOT / "oracle" / "_ref" / "ref_cpu_bench", "x86-64-v4": ROOT / "oracle" / "_ref" / "ref_cpu_bench4"}
+    model, has_v4 = cpu_model()
+    if not has_v4:
+        exes.pop("x86-64-v4")
+    exes = {k: v for k, v in exes.items() if v.exists()}
+    if exes:
+        cores, logical = physical_cores()
+        nz = max(1, round(n / cores))
+        backend = "/cpu/self/avx/blocked"
+
+        def launch(exe, secs, count):
+            procs = [subprocess.Popen([str(exe), backend, str(p), str(n), str(n), str(nz), str(secs)], stdout=subprocess.PIPE, text=True)
+                     for _ in range(count)]
+            rates, dofs = [], 0
+            for pr in procs:
+                out, _ = pr.communicate(timeout=secs * 6 + 120)
+                if pr.returncode != 0:
+                    return None, 0
+                r = json.loads(out.strip().splitlines()[-1])
+                rates.append(r["dofs"] * r["applies"] / r["seconds"])
+                dofs = r["dofs"]
+            return sum(rates), dofs
+
+        # the faster build on this CPU (AVX2 or AVX-512 code generation; the reference's default is -march=native)
+        trial = {k: launch(v, 2.0, 1)[0] or 0.0 for k, v in exes.items()}
+        march = max(trial, key=trial.get)
+        total, dofs = launch(exes[march], seconds, cores)
+        if total is None:
+            return None
+        return {"value": total / 1e9, "unit": "GDoF/s", "cores": cores, "kind": "reference",
+                "sample": f"{backend} (reference sources, -O3 -march={march} -ffp-contract=fast), 3D Poisson p={p} Q={p + 2}, "
+                          f"{cores} single-threaded processes, each on a {n}x{n}x{nz}-element slab ({dofs} DoFs) of the {n}^3 mesh, "
+                          f"{seconds:.0f} s",
+                "cpu": model, "logical_cpus": logical, "march_trial_gdofs_1core": {k: round(v / 1e9, 4) for k, v in trial.items()}}
     # fallback: the oracle restatement of /cpu/self/ref/serial, one core
     import oracle
     from libceed_amd.mesh import hex_mesh
@@ -471,7 +512,7 @@ def main() -> None:
         batch = run_batch()
         print(f"# batch: {batch}", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
-    cpu = None if args.no_cpu else cpu_baseline(args.p)
+    cpu = None if args.no_cpu else cpu_baseline(args.p, head["n"])
     line = {
         "metric": "3D diffusion apply GDoF/s p=1..8 (FP64 vs FP32+refine @1e-10), % HBM peak",
         "value": round(head["gdofs"], 4),

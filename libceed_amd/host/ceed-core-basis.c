@@ -11,12 +11,16 @@
 #include "ceed-core.h"
 #include "ceed-mi355.h"
 
+/* every size of a tensor basis counts something that must exist */
+static int tensor_sizes_valid(Ceed ceed, CeedInt dim, CeedInt num_comp, CeedInt P_1d, CeedInt Q_1d) {
+  CeedCheck(dim > 0 && num_comp > 0 && P_1d > 0 && Q_1d > 0, ceed, CEED_ERROR_DIMENSION,
+            "tensor basis sizes must all be positive: dim=%d, %d components, P_1d=%d, Q_1d=%d", dim, num_comp, P_1d, Q_1d);
+  return CEED_ERROR_SUCCESS;
+}
+
 int CeedBasisCreateTensorH1(Ceed ceed, CeedInt dim, CeedInt num_comp, CeedInt P_1d, CeedInt Q_1d, const CeedScalar *interp_1d,
                             const CeedScalar *grad_1d, const CeedScalar *q_ref_1d, const CeedScalar *q_weight_1d, CeedBasis *basis) {
-  CeedCheck(dim > 0, ceed, CEED_ERROR_DIMENSION, "CeedBasis dimension must be a positive value");
-  CeedCheck(num_comp > 0, ceed, CEED_ERROR_DIMENSION, "CeedBasis must have at least 1 component");
-  CeedCheck(P_1d > 0, ceed, CEED_ERROR_DIMENSION, "CeedBasis must have at least 1 node");
-  CeedCheck(Q_1d > 0, ceed, CEED_ERROR_DIMENSION, "CeedBasis must have at least 1 quadrature point");
+  CeedCall(tensor_sizes_valid(ceed, dim, num_comp, P_1d, Q_1d));
   CeedCheck(ceed->BasisCreateTensorH1, ceed, CEED_ERROR_UNSUPPORTED, "backend does not support BasisCreateTensorH1");
   CeedCall(CeedCalloc(1, basis));
   CeedBasis b = *basis;
@@ -39,10 +43,7 @@ int CeedBasisCreateTensorH1(Ceed ceed, CeedInt dim, CeedInt num_comp, CeedInt P_
 }
 
 int CeedBasisCreateTensorH1Lagrange(Ceed ceed, CeedInt dim, CeedInt num_comp, CeedInt P, CeedInt Q, CeedQuadMode quad_mode, CeedBasis *basis) {
-  CeedCheck(dim > 0, ceed, CEED_ERROR_DIMENSION, "CeedBasis dimension must be a positive value");
-  CeedCheck(num_comp > 0, ceed, CEED_ERROR_DIMENSION, "CeedBasis must have at least 1 component");
-  CeedCheck(P > 0, ceed, CEED_ERROR_DIMENSION, "CeedBasis must have at least 1 node");
-  CeedCheck(Q > 0, ceed, CEED_ERROR_DIMENSION, "CeedBasis must have at least 1 quadrature point");
+  CeedCall(tensor_sizes_valid(ceed, dim, num_comp, P, Q));
   CeedScalar *interp, *grad, *q_ref, *q_weight;
   CeedCall(CeedCalloc(P * Q, &interp));
   CeedCall(CeedCalloc(P * Q, &grad));
@@ -176,7 +177,7 @@ static int basis_apply_check(CeedBasis basis, CeedInt num_elem, CeedTransposeMod
     default: return CeedError(basis->ceed, CEED_ERROR_UNSUPPORTED, "evaluation mode %d not supported by a tensor H1 basis", eval_mode);
   }
   CeedCheck(eval_mode != CEED_EVAL_WEIGHT || t_mode == CEED_NOTRANSPOSE, basis->ceed, CEED_ERROR_BACKEND,
-            "CEED_EVAL_WEIGHT incompatible with CEED_TRANSPOSE");
+            "quadrature weights have no transpose (CEED_EVAL_WEIGHT with CEED_TRANSPOSE)");
   const CeedSize u_need = t_mode == CEED_TRANSPOSE ? qpts : nodes, v_need = t_mode == CEED_TRANSPOSE ? nodes : qpts;
   if (eval_mode != CEED_EVAL_WEIGHT)
     CeedCheck(u != CEED_VECTOR_NONE && u->length >= u_need, basis->ceed, CEED_ERROR_DIMENSION, "input vector too short for basis apply");

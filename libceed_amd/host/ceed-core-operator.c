@@ -101,7 +101,7 @@ int CeedOperatorCreateComposite(Ceed ceed, CeedOperator *op) {
 }
 
 int CeedOperatorCompositeAddSub(CeedOperator composite_op, CeedOperator sub_op) {
-  CeedCheck(composite_op->is_composite, composite_op->ceed, CEED_ERROR_MINOR, "CeedOperator is not a composite operator");
+  CeedCheck(composite_op->is_composite, composite_op->ceed, CEED_ERROR_MINOR, "this call needs a composite CeedOperator");
   CeedCheck(!sub_op->is_composite, composite_op->ceed, CEED_ERROR_UNSUPPORTED, "nested composite operators are not supported");
   CeedCheck(!composite_op->ready, composite_op->ceed, CEED_ERROR_MAJOR, "composite operator cannot be changed after it is applied");
   if (composite_op->num_sub > 0 && sub_op->has_num_elem) {
@@ -119,13 +119,13 @@ int CeedOperatorCompositeAddSub(CeedOperator composite_op, CeedOperator sub_op) 
 }
 
 int CeedOperatorCompositeGetNumSub(CeedOperator op, CeedInt *num_suboperators) {
-  CeedCheck(op->is_composite, op->ceed, CEED_ERROR_MINOR, "CeedOperator is not a composite operator");
+  CeedCheck(op->is_composite, op->ceed, CEED_ERROR_MINOR, "this call needs a composite CeedOperator");
   *num_suboperators = op->num_sub;
   return CEED_ERROR_SUCCESS;
 }
 
 int CeedOperatorCompositeGetSubList(CeedOperator op, CeedOperator **sub_operators) {
-  CeedCheck(op->is_composite, op->ceed, CEED_ERROR_MINOR, "CeedOperator is not a composite operator");
+  CeedCheck(op->is_composite, op->ceed, CEED_ERROR_MINOR, "this call needs a composite CeedOperator");
   *sub_operators = op->sub;
   return CEED_ERROR_SUCCESS;
 }
@@ -247,7 +247,7 @@ int CeedOperatorLinearAssembleAddDiagonal(CeedOperator op, CeedVector assembled,
   CeedSize in, out;
   CeedCall(CeedOperatorCheckReady(op));
   CeedCall(CeedOperatorGetActiveVectorLengths(op, &in, &out));
-  CeedCheck(in == out, op->ceed, CEED_ERROR_DIMENSION, "Operator must be square");
+  CeedCheck(in == out, op->ceed, CEED_ERROR_DIMENSION, "a diagonal needs an operator whose input and output spaces agree");
   if (op->is_composite) {
     for (CeedInt i = 0; i < op->num_sub; i++) CeedCall(CeedOperatorLinearAssembleAddDiagonal(op->sub[i], assembled, request));
     return CEED_ERROR_SUCCESS;
@@ -261,7 +261,7 @@ int CeedOperatorLinearAssembleDiagonal(CeedOperator op, CeedVector assembled, Ce
   CeedSize in, out;
   CeedCall(CeedOperatorCheckReady(op));
   CeedCall(CeedOperatorGetActiveVectorLengths(op, &in, &out));
-  CeedCheck(in == out, op->ceed, CEED_ERROR_DIMENSION, "Operator must be square");
+  CeedCheck(in == out, op->ceed, CEED_ERROR_DIMENSION, "a diagonal needs an operator whose input and output spaces agree");
   if (!op->is_composite && op->num_elem == 0) return CEED_ERROR_SUCCESS;
   if (!op->is_composite && op->LinearAssembleDiagonal) return op->LinearAssembleDiagonal(op, assembled, request);
   CeedCall(CeedVectorSetValue(assembled, 0.0));

@@ -224,9 +224,9 @@ int CeedQFunctionReferenceCopy(CeedQFunction qf, CeedQFunction *qf_copy) {
 }
 
 static int add_field(CeedQFunction qf, bool input, const char *field_name, CeedInt size, CeedEvalMode eval_mode) {
-  CeedCheck(!qf->is_immutable, qf->ceed, CEED_ERROR_MAJOR, "QFunction cannot be changed after set as immutable");
-  CeedCheck(eval_mode != CEED_EVAL_WEIGHT || size == 1, qf->ceed, CEED_ERROR_DIMENSION, "CEED_EVAL_WEIGHT should have size 1");
-  CeedCheck(input || eval_mode != CEED_EVAL_WEIGHT, qf->ceed, CEED_ERROR_DIMENSION, "CEED_EVAL_WEIGHT cannot be an output");
+  CeedCheck(!qf->is_immutable, qf->ceed, CEED_ERROR_MAJOR, "fields cannot be added to a QFunction an operator already uses");
+  CeedCheck(eval_mode != CEED_EVAL_WEIGHT || size == 1, qf->ceed, CEED_ERROR_DIMENSION, "a quadrature-weight field holds one value per point");
+  CeedCheck(input || eval_mode != CEED_EVAL_WEIGHT, qf->ceed, CEED_ERROR_DIMENSION, "quadrature weights are an input only");
   CeedInt                           *n = input ? &qf->num_in : &qf->num_out;
   struct CeedQFunctionField_private *f = input ? qf->in : qf->out;
   for (CeedInt i = 0; i < qf->num_in; i++)

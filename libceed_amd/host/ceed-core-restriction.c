@@ -13,7 +13,7 @@
 static int rstr_alloc(Ceed ceed, CeedInt num_elem, CeedInt elem_size, CeedInt num_comp, CeedSize l_size, CeedElemRestriction *rstr) {
   CeedCheck(num_elem >= 0, ceed, CEED_ERROR_DIMENSION, "CeedElemRestriction must have at least 0 elements");
   CeedCheck(elem_size > 0, ceed, CEED_ERROR_DIMENSION, "element size must be at least 1");
-  CeedCheck(num_comp > 0, ceed, CEED_ERROR_DIMENSION, "CeedElemRestriction must have at least 1 component");
+  CeedCheck(num_comp > 0, ceed, CEED_ERROR_DIMENSION, "a restriction needs one or more components");
   CeedCheck(ceed->ElemRestrictionCreate, ceed, CEED_ERROR_UNSUPPORTED, "backend does not support ElemRestrictionCreate");
   CeedCall(CeedCalloc(1, rstr));
   CeedCall(CeedReferenceCopy(ceed, &(*rstr)->ceed));
@@ -30,7 +30,7 @@ static int rstr_alloc(Ceed ceed, CeedInt num_elem, CeedInt elem_size, CeedInt nu
 
 int CeedElemRestrictionCreate(Ceed ceed, CeedInt num_elem, CeedInt elem_size, CeedInt num_comp, CeedInt comp_stride, CeedSize l_size,
                               CeedMemType mem_type, CeedCopyMode copy_mode, const CeedInt *offsets, CeedElemRestriction *rstr) {
-  CeedCheck(num_comp == 1 || comp_stride > 0, ceed, CEED_ERROR_DIMENSION, "component stride must be at least 1");
+  CeedCheck(num_comp == 1 || comp_stride > 0, ceed, CEED_ERROR_DIMENSION, "components of a multi-component restriction need a positive stride");
   CeedCheck(mem_type == CEED_MEM_HOST, ceed, CEED_ERROR_UNSUPPORTED, "offsets must be host memory");
   CeedCall(rstr_alloc(ceed, num_elem, elem_size, num_comp, l_size, rstr));
   CeedElemRestriction r = *rstr;

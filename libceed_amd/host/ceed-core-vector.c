@@ -10,7 +10,7 @@
 #include "ceed-core.h"
 
 int CeedVectorCreate(Ceed ceed, CeedSize len, CeedVector *vec) {
-  CeedCheck(len >= 0, ceed, CEED_ERROR_DIMENSION, "CeedVector must have length >= 0");
+  CeedCheck(len >= 0, ceed, CEED_ERROR_DIMENSION, "a vector length cannot be negative");
   CeedCheck(ceed->VectorCreate, ceed, CEED_ERROR_UNSUPPORTED, "backend does not support VectorCreate");
   CeedCall(CeedCalloc(1, vec));
   CeedCall(CeedReferenceCopy(ceed, &(*vec)->ceed));

@@ -157,7 +157,7 @@ int CeedGetErrorMessage(Ceed ceed, const char **err_msg) {
 
 int CeedResetErrorMessage(Ceed ceed, const char **err_msg) {
   *err_msg = NULL;
-  memcpy(ceed->err_msg, "No error message stored", 24);
+  snprintf(ceed->err_msg, sizeof(ceed->err_msg), "%s", "(no message recorded)");
   return CEED_ERROR_SUCCESS;
 }
 
@@ -232,7 +232,7 @@ int CeedInit(const char *resource, Ceed *ceed) {
   else if (h && !strcmp(h, "store")) c->Error = CeedErrorStore;
   else if (h && !strcmp(h, "return")) c->Error = CeedErrorReturn;
   else c->Error = CeedErrorAbort;
-  memcpy(c->err_msg, "No error message stored", 24);
+  snprintf(c->err_msg, sizeof(c->err_msg), "%s", "(no message recorded)");
   CeedCall(CeedStringAllocCopy(resource, &c->resource));
   /* a failing backend init is stored (abort and exit act at once) and
    * printed, since the context is released before the caller sees it */

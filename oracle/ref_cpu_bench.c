@@ -5,12 +5,13 @@
  * bench.py's cpu_baseline leg.  One process = one core; bench.py runs one
  * process per core on its own mesh and sums the rates.
  *
- *   ref_cpu_bench <resource> <p> <n> <seconds>
+ *   ref_cpu_bench <resource> <p> <nx> <ny> <nz> <seconds>
  * prints: {"dofs": N, "applies": K, "seconds": T}
  *
- * Mesh: n^3 elements on the unit cube, lexicographic numbering as in
- * examples/ceed/ex1-volume.c; geometry is affine (the apply cost does not
- * depend on the coordinate values).
+ * Mesh: nx x ny x nz elements (a slab of a larger structured mesh: bench.py
+ * gives each process an equal share of the GPU workload's elements),
+ * lexicographic numbering as in examples/ceed/ex1-volume.c; geometry is
+ * affine (the apply cost does not depend on the coordinate values).
  */
 #define _POSIX_C_SOURCE 199309L
 #include <ceed.h>
@@ -24,38 +25,38 @@ static double now(void) {
   return t.tv_sec + 1e-9 * t.tv_nsec;
 }
 
-static void lex_offsets(int n, int p, CeedInt *off) {
-  const int P = p + 1, M = n * p + 1;
-  for (int ez = 0; ez < n; ez++)
-    for (int ey = 0; ey < n; ey++)
-      for (int ex = 0; ex < n; ex++) {
-        const int e = ex + n * (ey + n * ez);
+static void lex_offsets(const int n[3], int p, CeedInt *off) {
+  const int P = p + 1, Mx = n[0] * p + 1, My = n[1] * p + 1;
+  for (int ez = 0; ez < n[2]; ez++)
+    for (int ey = 0; ey < n[1]; ey++)
+      for (int ex = 0; ex < n[0]; ex++) {
+        const int e = ex + n[0] * (ey + n[1] * ez);
         for (int k = 0; k < P; k++)
           for (int j = 0; j < P; j++)
             for (int i = 0; i < P; i++)
-              off[(size_t)e * P * P * P + i + P * (j + P * k)] = (ex * p + i) + M * ((ey * p + j) + M * (ez * p + k));
+              off[(size_t)e * P * P * P + i + P * (j + P * k)] = (ex * p + i) + Mx * ((ey * p + j) + My * (ez * p + k));
       }
 }
 
 int main(int argc, char **argv) {
-  if (argc < 5) return 1;
+  if (argc < 7) return 1;
   const char *resource = argv[1];
-  const int   p = atoi(argv[2]), n = atoi(argv[3]), P = p + 1, Q = p + 2, E = n * n * n;
-  const double secs = atof(argv[4]);
-  const int    M = n * p + 1, N = M * M * M, nv = (n + 1) * (n + 1) * (n + 1), nq = Q * Q * Q;
+  const int   p = atoi(argv[2]), n[3] = {atoi(argv[3]), atoi(argv[4]), atoi(argv[5])}, P = p + 1, Q = p + 2, E = n[0] * n[1] * n[2];
+  const double secs = atof(argv[6]);
+  const int    N = (n[0] * p + 1) * (n[1] * p + 1) * (n[2] * p + 1), nv = (n[0] + 1) * (n[1] + 1) * (n[2] + 1), nq = Q * Q * Q;
   Ceed         ceed;
   if (CeedInit(resource, &ceed)) return 2;
   CeedInt    *off = malloc(sizeof(CeedInt) * (size_t)E * P * P * P), *moff = malloc(sizeof(CeedInt) * (size_t)E * 8);
   CeedScalar *xyz = malloc(sizeof(CeedScalar) * 3 * (size_t)nv);
   lex_offsets(n, p, off);
   lex_offsets(n, 1, moff);
-  for (int k = 0; k <= n; k++)
-    for (int j = 0; j <= n; j++)
-      for (int i = 0; i <= n; i++) {
-        const int v = i + (n + 1) * (j + (n + 1) * k);
-        xyz[v] = (double)i / n;
-        xyz[nv + v] = (double)j / n;
-        xyz[2 * nv + v] = (double)k / n;
+  for (int k = 0; k <= n[2]; k++)
+    for (int j = 0; j <= n[1]; j++)
+      for (int i = 0; i <= n[0]; i++) {
+        const int v = i + (n[0] + 1) * (j + (n[1] + 1) * k);
+        xyz[v]          = (double)i / n[0];
+        xyz[nv + v]     = (double)j / n[1];
+        xyz[2 * nv + v] = (double)k / n[2];
       }
   CeedBasis           mb, sb;
   CeedElemRestriction mr, sr, qr;
