@@ -161,3 +161,33 @@ def hex_kappa_gradient(num_elem, P, Q, offsets, q_data, u, lam, quad_mode=GAUSS)
     _ok(lib().orc_hex_kappa_gradient(num_elem, P, Q, quad_mode, _p(np.ascontiguousarray(offsets, dtype=np.int32)), _p(q_data),
                                      _p(u), _p(lam), _p(dk)))
     return dk
+
+
+# gallery name -> (oracle function, input sizes per point, output sizes per point)
+QF_GALLERY = {
+    "Mass3DBuild": ("orc_qf_mass3d_build", (9, 1), (1,)),
+    "MassApply": ("orc_qf_mass_apply", (1, 1), (1,)),
+    "Poisson3DBuild": ("orc_qf_poisson3d_build", (9, 1), (6,)),
+    "Poisson3DApply": ("orc_qf_poisson3d_apply", (3, 6), (3,)),
+    "Vector3MassApply": ("orc_qf_vector3_mass_apply", (3, 1), (3,)),
+    "Vector3Poisson3DApply": ("orc_qf_vector3_poisson3d_apply", (9, 6), (9,)),
+}
+
+
+def qf_apply(name, Q, inputs):
+    """The oracle's pointwise gallery QFunction ``name`` over Q points.
+
+    ``inputs`` are arrays of size * Q values each (field component k of point
+    i at k * Q + i, the layout CeedQFunctionApply hands a QFunction); returns
+    the list of output arrays in the same layout."""
+    fname, in_sizes, out_sizes = QF_GALLERY[name]
+    ins = [np.ascontiguousarray(a, dtype=np.float64) for a in inputs]
+    assert [a.size for a in ins] == [s * Q for s in in_sizes], "input sizes"
+    outs = [np.zeros(s * Q) for s in out_sizes]
+    f = getattr(lib(), fname)
+    f.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    f.restype = C.c_int
+    in_ptrs = (C.c_void_p * len(ins))(*[a.ctypes.data for a in ins])
+    out_ptrs = (C.c_void_p * len(outs))(*[a.ctypes.data for a in outs])
+    _ok(f(None, Q, in_ptrs, out_ptrs))
+    return outs

@@ -9,8 +9,10 @@ import numpy as np
 import pytest
 import torch
 
+import oracle
 from libceed_amd import ops
-from libceed_amd.batch import BatchCG, random_systems
+from libceed_amd.batch import BatchCG, System, random_systems
+from libceed_amd.mesh import hex_mesh
 
 pytestmark = pytest.mark.gpu
 
@@ -81,3 +83,52 @@ def test_batch_scatter_modes(cuda, scatter):
         if s.kind == ops.QF_DIFFUSION:
             d = d - d.mean()
         assert (torch.linalg.norm(d) / torch.linalg.norm(x_ref[r])).item() < 1e-8, (scatter, i)
+
+
+def dense_operator(kind, mesh):
+    """The oracle's operator as a dense matrix: column j = A e_j (oracle q_data, Q = P + 1)."""
+    Q = mesh.P + 1
+    qd = oracle.hex_build_qdata(0 if kind == ops.QF_MASS else 1, mesh, Q)
+    N = mesh.num_nodes
+    A = np.zeros((N, N))
+    e = np.zeros(N)
+    for j in range(N):
+        e[j] = 1.0
+        A[:, j] = oracle.hex_apply(0 if kind == ops.QF_MASS else 1, 1, mesh.num_elem, mesh.P, Q, mesh.offsets, N, qd, e)
+        e[j] = 0.0
+    return A
+
+
+def oracle_solution(kind, mesh, b):
+    """Mass: A^-1 b.  Diffusion (pure Neumann, null space = constants, b orthogonal to them): the
+    minimum-norm solution, (A + 11^T/N)^-1 b, which is orthogonal to the constants."""
+    A = dense_operator(kind, mesh)
+    if kind == ops.QF_DIFFUSION:
+        A = A + 1.0 / mesh.num_nodes
+    return np.linalg.solve(A, b)
+
+
+def test_batch_every_group_vs_oracle(cuda):
+    """A batch holding every (mass or diffusion, p = 2..7) group of the batched workload, two meshes of
+    different size per group (one of them a single element): every system's batched solution against the
+    oracle's dense solve, at 1e-8 (diffusion compared modulo constants)."""
+    systems = []
+    for p in range(2, 8):
+        for kind in (ops.QF_MASS, ops.QF_DIFFUSION):
+            for n in (1, 2 if p >= 5 else 3):
+                systems.append(System(kind, hex_mesh(n, p, seed=97 * p + 13 * n + kind)))
+    bcg = BatchCG(systems)
+    assert {(s.kind, s.mesh.P) for s in systems} == {(k, p + 1) for k in (ops.QF_MASS, ops.QF_DIFFUSION) for p in range(2, 8)}
+    b = batch_rhs(bcg, seed=4)
+    x = torch.zeros_like(b)
+    st = bcg.solve(b, x, max_iter=5000, rtol=1e-12, check_every=10)
+    assert np.all(st["rel_residual"] <= 1e-12), st["rel_residual"]
+    xh, bh = x.cpu().numpy(), b.cpu().numpy()
+    for i, s in enumerate(systems):
+        r = bcg.range(i)
+        print("batch vs oracle", i, s.kind, s.mesh.n, s.mesh.p, flush=True)
+        ref = oracle_solution(s.kind, s.mesh, bh[r])
+        d = xh[r] - ref
+        if s.kind == ops.QF_DIFFUSION:
+            d = d - d.mean()
+        assert np.linalg.norm(d) / np.linalg.norm(ref) < 1e-8, (i, s.kind, s.mesh.n, s.mesh.p)

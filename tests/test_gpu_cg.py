@@ -183,3 +183,31 @@ def test_pcg_jacobi_converges(cuda, mixed):
     x2 = torch.zeros_like(b)
     st2 = solver.solve(b, x2, max_iter=5000, rtol=1e-10, check_every=5)
     assert st2.iterations > st.iterations
+
+
+@pytest.mark.parametrize("p,n", [(4, 3), (6, 2)])
+def test_cg_mixed_precision_vs_oracle_solution(cuda, p, n):
+    """The mixed-precision pattern (FP32 inner CG, FP64 residual replacement every k in {10, 25, 50}) at the
+    orders of the mixed-precision workload, on meshes the oracle solves densely: rhs = A u_exact with the
+    oracle's operator, solution against the oracle's minimum-norm solution (pure Neumann: modulo constants);
+    relative residual 1e-10 and error below 1e-8 (the error a 1e-10 residual allows at these
+    condition numbers; the printed errors are what is reported)."""
+    m = hex_mesh(n, p, seed=17 + p)
+    Q = p + 2
+    qd = oracle.hex_build_qdata(1, m, Q)
+    A = dense_operator(m, Q, qd)
+    xyz = node_coordinates(m, oracle.lobatto(m.P)[0])
+    u = np.sin(np.pi * xyz[0]) * np.sin(np.pi * xyz[1]) * np.sin(np.pi * xyz[2])
+    bh = A @ u
+    x_ref = np.linalg.solve(A + 1.0 / m.num_nodes, bh)  # b is orthogonal to the constants: the min-norm solution
+    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q)
+    op32 = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q, dtype=torch.float32)
+    b = torch.from_numpy(bh).cuda()
+    for k in (10, 25, 50):
+        x = torch.zeros_like(b)
+        st = ops.CG(op32, op).solve(b, x, max_iter=6000, rtol=1e-10, check_every=k)
+        assert st.rel_residual <= 1e-10, (k, st.rel_residual)
+        d = x.cpu().numpy() - x_ref
+        err = np.linalg.norm(d - d.mean()) / np.linalg.norm(x_ref)
+        print(f"mixed CG p={p} k={k}: {st.iterations} iterations, error vs oracle solution {err:.2e}", flush=True)
+        assert err < 1e-8, (k, err)

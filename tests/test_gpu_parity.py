@@ -405,3 +405,46 @@ def test_operator_no_elements(cuda, scatter):
     w = torch.full((10,), 2.0, dtype=torch.float64, device="cuda")
     op.apply_add(torch.ones(10, dtype=torch.float64, device="cuda"), w)
     assert torch.all(w == 2.0).item()
+
+
+_HEADLINE = {}
+
+
+def headline_reference(p):
+    """The ~8M-DoF diffusion workload (p=4: 50^3 elements, 8.12M DoFs; p=6: 33^3, 7.88M), Q = p + 2: mesh,
+    seeded u, the oracle's q_data and the oracle's v = A u (about half a minute of host time, done once)."""
+    if p not in _HEADLINE:
+        from libceed_amd.mesh import mesh_size_for_dofs
+
+        m = hex_mesh(mesh_size_for_dofs(p, 8e6), p, seed=41)
+        Q = p + 2
+        u = np.random.default_rng(p).uniform(-1, 1, m.num_nodes)
+        qd = oracle.hex_build_qdata(1, m, Q)
+        _HEADLINE[p] = (m, Q, u, qd, oracle.hex_apply(1, 1, m.num_elem, p + 1, Q, m.offsets, m.num_nodes, qd, u))
+    return _HEADLINE[p]
+
+
+@pytest.mark.parametrize("p,scatter", [(4, "auto"), (4, "brick"), (6, "auto"), (6, "brick")])
+def test_headline_workload_vs_oracle(cuda, p, scatter):
+    """The benchmarked FP64 apply at full size against the oracle at 1e-12: q_data built on the GPU, then
+    Apply and ApplyAdd with the library's default E^T (colours at p=4, the multiplicity split at p=6) and
+    with bricks."""
+    m, Q, u, qd, v_ref = headline_reference(p)
+    print("headline vs oracle p", p, scatter, "dofs", m.num_nodes, flush=True)
+    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q, scatter=scatter)
+    assert rel_l2(host(op.get_qdata()), qd) < RTOL64
+    v = torch.full((m.num_nodes,), 5.0, dtype=torch.float64, device="cuda")
+    ud = dev(u)
+    op.apply(ud, v)
+    assert rel_l2(host(v), v_ref) < RTOL64
+    op.apply_add(ud, v)
+    assert rel_l2(host(v), 2.0 * v_ref) < RTOL64
+
+
+def test_headline_workload_fp32_vs_oracle(cuda):
+    """The FP32 operator (FP32 q_data, vectors and accumulation) at 8.12M DoFs, p=4, against the FP64 oracle."""
+    m, Q, u, qd, v_ref = headline_reference(4)
+    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q, dtype=torch.float32)
+    v = torch.zeros(m.num_nodes, dtype=torch.float32, device="cuda")
+    op.apply(dev(u, torch.float32), v)
+    assert rel_l2(host(v), v_ref) < RTOL32

@@ -238,3 +238,35 @@ def test_live_reference_mass_config(tmp_path):
     v = oracle.hex_apply(0, 1, m.num_elem, 4, 5, m.offsets, m.num_nodes, qd, u)
     assert np.array_equal(qd, np.fromfile(str(pre) + ".qdata"))
     assert np.array_equal(v, np.fromfile(str(pre) + ".v"))
+
+
+def test_pointwise_qfunctions_closed_forms():
+    """The oracle's gallery QFunctions against their mathematics at random points: Mass3DBuild = w det J,
+    Poisson3DBuild = w det J J^-1 J^-T (J = dx/dX, stored transposed) in Voigt order {xx, yy, zz, yz, xz, xy}, the apply functions as
+    the products they define (a wrong Voigt index or a transposed J fails here)."""
+    rng = np.random.default_rng(11)
+    Q = 7
+    J = np.eye(3)[:, :, None] + 0.4 * rng.uniform(-1, 1, (3, 3, Q))
+    w = rng.uniform(0.1, 1, Q)
+    Jl = [J[:, :, i] for i in range(Q)]
+    (qm,) = oracle.qf_apply("Mass3DBuild", Q, [J.reshape(-1), w])
+    assert np.allclose(qm, [np.linalg.det(a) * wi for a, wi in zip(Jl, w)], rtol=1e-13, atol=0)
+    (qp,) = oracle.qf_apply("Poisson3DBuild", Q, [J.reshape(-1), w])
+    voigt = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    for i, (a, wi) in enumerate(zip(Jl, w)):
+        inv = np.linalg.inv(a)  # a[d][c] = dx_c/dX_d (GRAD layout [dir][comp]): the Jacobian is a^T
+        S = wi * np.linalg.det(a) * inv.T @ inv
+        assert np.allclose(qp.reshape(6, Q)[:, i], [S[r, c] for r, c in voigt], rtol=1e-12, atol=1e-14)
+    # applies: S from the Voigt entries, ug stored [dir][Q] (scalar) or [dir][comp][Q] (vector)
+    S = np.zeros((Q, 3, 3))
+    for k, (r, c) in enumerate(voigt):
+        S[:, r, c] = S[:, c, r] = qp.reshape(6, Q)[k]
+    ug = rng.uniform(-1, 1, (3, Q))
+    (vg,) = oracle.qf_apply("Poisson3DApply", Q, [ug.reshape(-1), qp])
+    assert np.allclose(vg.reshape(3, Q), np.einsum("qij,iq->jq", S, ug), rtol=1e-13, atol=1e-15)
+    ug3 = rng.uniform(-1, 1, (3, 3, Q))
+    (vg3,) = oracle.qf_apply("Vector3Poisson3DApply", Q, [ug3.reshape(-1), qp])
+    assert np.allclose(vg3.reshape(3, 3, Q), np.einsum("qij,icq->jcq", S, ug3), rtol=1e-13, atol=1e-15)
+    u, u3 = rng.uniform(-1, 1, Q), rng.uniform(-1, 1, (3, Q))
+    assert np.array_equal(oracle.qf_apply("MassApply", Q, [u, qm])[0], u * qm)
+    assert np.array_equal(oracle.qf_apply("Vector3MassApply", Q, [u3.reshape(-1), qm])[0], (u3 * qm).reshape(-1))
