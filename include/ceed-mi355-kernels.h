@@ -240,6 +240,15 @@ int CeedMi355VectorOp(int op, int scalar_type, int64_t n, double alpha, double b
  * fixed-order tree, so equal inputs give equal bits.  Replaces CeedVectorNorm
  * (interface/ceed-vector.c:677-733) and the dot products of the CG drivers. */
 int CeedMi355VectorReduce(int op, int scalar_type, int64_t n, const void *x, const void *y, double *result_device, void *stream);
+/* CeedMi355VectorReduce without any allocation, the same kernels and bits.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The block partials live in a fixed array of 1024
+ * doubles (8 KiB) in each GPU's memory, held until the process ends; every n
+ * fits it (the reduction grid is capped at 1024 blocks).  All calls on one GPU
+ * share that array, so calls that overlap on one GPU race.  Nothing in the
+ * array needs clearing: each call writes every slot it reads. */
+int CeedMi355VectorReduceNoAlloc(int op, int scalar_type, int64_t n, const void *x, const void *y, double *result_device, void *stream);
 /* y = (to_type) x, element by element (FP64 <-> FP32). */
 int CeedMi355VectorConvert(int64_t n, int from_type, const void *x, int to_type, void *y, void *stream);
 

@@ -50,6 +50,7 @@ SIGNATURES = {
     "CeedMi355LagrangeBasis1D": (c_int, [c_i32, c_i32, c_i32, vp, vp, vp, vp]),
     "CeedMi355VectorOp": (c_int, [c_int, c_int, c_i64, C.c_double, C.c_double, vp, vp, vp, vp]),
     "CeedMi355VectorReduce": (c_int, [c_int, c_int, c_i64, vp, vp, vp, vp]),
+    "CeedMi355VectorReduceNoAlloc": (c_int, [c_int, c_int, c_i64, vp, vp, vp, vp]),
     "CeedMi355VectorConvert": (c_int, [c_i64, c_int, vp, c_int, vp, vp]),
     "CeedMi355QFunctionCompile": (c_int, [C.c_char_p, C.c_char_p, c_i32, vp, c_i32, vp, C.POINTER(vp)]),
     "CeedMi355QFunctionApply": (c_int, [vp, c_i64, c_i32, vp, vp, vp, vp]),

@@ -29,7 +29,8 @@ extern "C" int CeedMi355DeviceInit(int device_id) {
   MI355_HIP(hipGetDeviceProperties(&prop, device_id));
   MI355_CHECK(strncmp(prop.gcnArchName, "gfx950", 6) == 0, CEED_MI355_ERROR_UNSUPPORTED,
               "device %d is %s; this library is built for gfx950 (MI355X) only", device_id, prop.gcnArchName);
-  return CEED_MI355_SUCCESS;
+  // the device address of this GPU's no-allocation reduction scratch (vector.hip)
+  return mi355::reduce_scratch_setup(nullptr);
 }
 
 extern "C" int CeedMi355Malloc(size_t bytes, void **ptr) {

@@ -12,6 +12,11 @@ namespace mi355 {
 // Last error message of the calling host thread (read by CeedMi355GetLastError).
 void set_error(const char *fmt, ...);
 
+// The current device's address of the fixed scratch array of
+// CeedMi355VectorReduceNoAlloc (vector.hip), looked up on the first call for
+// a device and kept; ptr may be nullptr (set-up only).
+int reduce_scratch_setup(double **ptr);
+
 }  // namespace mi355
 
 #define MI355_CHECK(cond, code, ...)    \

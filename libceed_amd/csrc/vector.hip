@@ -88,16 +88,24 @@ double *partials(int &ierr) {
 }
 
 template <typename T>
-int reduce(int op, int64_t n, const T *x, const T *y, double *result_device, hipStream_t s) {
-  int     ierr = 0;
-  double *p    = partials(ierr);
-  MI355_CHECK(!ierr, CEED_MI355_ERROR_BACKEND, "reduction scratch allocation failed");
+int reduce(int op, int64_t n, const T *x, const T *y, double *result_device, double *partial, hipStream_t s) {
   const int nb = mi355_blocks(n, kThreads, kReduceBlocks);
-  k_reduce1<T><<<nb, kThreads, 0, s>>>(op, n, x, y, p);
-  k_reduce2<<<1, kThreads, 0, s>>>(op, nb, p, result_device);
+  k_reduce1<T><<<nb, kThreads, 0, s>>>(op, n, x, y, partial);
+  k_reduce2<<<1, kThreads, 0, s>>>(op, nb, partial, result_device);
   MI355_LAUNCH_CHECK();
   return CEED_MI355_SUCCESS;
 }
+
+// The partial sums of CeedMi355VectorReduceNoAlloc: one fixed array in device
+// memory, which every GPU the library runs on holds (its own copy, at its own
+// address) until the process ends.  Host code reaches it only through the
+// per-device address hipGetSymbolAddress returns: the array's name is a
+// host-side stand-in, not a device address.  Every call on one GPU shares its
+// copy, so calls that overlap on one GPU race.
+constexpr int kScratchPartials = kReduceBlocks;
+static_assert(kScratchPartials >= kReduceBlocks, "one partial per block of the reduction grid");
+__device__ __align__(256) double scratch[kScratchPartials];
+double *scratch_ptr[64];  // per device id: the address of that GPU's copy, nullptr until set up
 
 template <typename T>
 int run_vector_op(int op, int64_t n, double a, double b, const void *x, const void *y, void *z, hipStream_t s) {
@@ -117,6 +125,22 @@ int run_vector_op(int op, int64_t n, double a, double b, const void *x, const vo
 
 }  // namespace
 
+// The current device's copy of the no-allocation scratch array: its address,
+// looked up once per device (hipGetSymbolAddress allocates nothing).
+int mi355::reduce_scratch_setup(double **ptr) {
+  int dev = 0;
+  MI355_HIP(hipGetDevice(&dev));
+  MI355_CHECK(dev >= 0 && dev < 64, CEED_MI355_ERROR_BACKEND, "device id %d outside the scratch table", dev);
+  if (!scratch_ptr[dev]) {
+    void *p = nullptr;
+    MI355_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MI355_CHECK(p != nullptr, CEED_MI355_ERROR_BACKEND, "no device address for the reduction scratch array on device %d", dev);
+    scratch_ptr[dev] = (double *)p;
+  }
+  if (ptr) *ptr = scratch_ptr[dev];
+  return CEED_MI355_SUCCESS;
+}
+
 extern "C" int CeedMi355VectorOp(int op, int scalar_type, int64_t n, double alpha, double beta, const void *x, const void *y, void *z,
                                  void *stream) {
   if (n <= 0) return CEED_MI355_SUCCESS;
@@ -133,8 +157,34 @@ extern "C" int CeedMi355VectorReduce(int op, int scalar_type, int64_t n, const v
     MI355_LAUNCH_CHECK();
     return CEED_MI355_SUCCESS;
   }
-  return scalar_type == CEED_MI355_FP64 ? reduce<double>(op, n, (const double *)x, (const double *)y, result_device, s)
-                                        : reduce<float>(op, n, (const float *)x, (const float *)y, result_device, s);
+  int     ierr = 0;
+  double *p    = partials(ierr);
+  MI355_CHECK(!ierr, CEED_MI355_ERROR_BACKEND, "reduction scratch allocation failed");
+  return scalar_type == CEED_MI355_FP64 ? reduce<double>(op, n, (const double *)x, (const double *)y, result_device, p, s)
+                                        : reduce<float>(op, n, (const float *)x, (const float *)y, result_device, p, s);
+}
+
+extern "C" int CeedMi355VectorReduceNoAlloc(int op, int scalar_type, int64_t n, const void *x, const void *y, double *result_device,
+                                            void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  MI355_CHECK(op >= 0 && op <= 2, CEED_MI355_ERROR_BACKEND, "unknown reduction %d", op);
+  MI355_CHECK(scalar_type == CEED_MI355_FP64 || scalar_type == CEED_MI355_FP32, CEED_MI355_ERROR_BACKEND, "unknown scalar type %d",
+              scalar_type);
+  if (n <= 0) {
+    k_set<double><<<1, kThreads, 0, s>>>(1, 0.0, result_device);
+    MI355_LAUNCH_CHECK();
+    return CEED_MI355_SUCCESS;
+  }
+  // the partials take one slot per block of the reduction grid, which is
+  // capped at kReduceBlocks blocks: every n fits the fixed array
+  const int nb = mi355_blocks(n, kThreads, kReduceBlocks);
+  MI355_CHECK(nb <= kScratchPartials, CEED_MI355_ERROR_BACKEND, "reduction of %lld entries needs %d partial slots, the scratch array holds %d",
+              (long long)n, nb, kScratchPartials);
+  double   *p    = nullptr;
+  const int ierr = mi355::reduce_scratch_setup(&p);  // first call on this device only; later calls read the table
+  if (ierr) return ierr;
+  return scalar_type == CEED_MI355_FP64 ? reduce<double>(op, n, (const double *)x, (const double *)y, result_device, p, s)
+                                        : reduce<float>(op, n, (const float *)x, (const float *)y, result_device, p, s);
 }
 
 extern "C" int CeedMi355VectorConvert(int64_t n, int from_type, const void *x, int to_type, void *y, void *stream) {
