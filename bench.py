@@ -65,6 +65,10 @@ def time_gpu(fn, steps: int, warmup: int):
     return s.elapsed_time(e) / steps, wall
 
 
+LAYOUT_NAMES = ["columns", "lines"]  # the fused kernel's thread layouts, by their index in the library
+KERNEL_OF_LAYOUT = ["hex_apply_kernel", "hex_apply_lines_kernel"]
+
+
 def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0, scatter: str = "auto", epb: int = 0, num_comp: int = 1,
              layout: int = 1):
     import torch
@@ -91,10 +95,10 @@ def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0,
     mode = {"tmap": "transpose map", "color": "element colours", "split": "multiplicity split", "atomic": "atomics",
             "brick": "element bricks"}[op.scatter_mode()]
     F = alg_flops(m.num_elem, P, Q, ncomp=num_comp)
-    kname = "hex_apply_lines_kernel" if layout == 1 else "hex_apply_kernel"
+    kname = KERNEL_OF_LAYOUT[layout]
     res = {
         "p": p, "n": n, "dofs": num_comp * m.num_nodes, "num_comp": num_comp, "elements": m.num_elem,
-        "scatter": mode, "layout": "lines" if layout == 1 else "columns", "elems_per_block": epb,
+        "scatter": mode, "layout": LAYOUT_NAMES[layout], "elems_per_block": epb,
         "kernels": {"transpose map": f"one apply = {kname} + transpose_gather_kernel (E^T by the transpose offset map)",
                     "multiplicity split": f"one apply = zero fill + {kname} + split_gather_kernel (E^T: atomic adds for entries of "
                                           "at most two contributions, the ordered transpose map for the rest)",
@@ -267,50 +271,59 @@ def run_drop_in(p: int = 4, n: int = 50, reps: int = 50, num_comp: int = 1, user
 
 
 def run_batch(count: int = 256, rtol: float = 1e-8, seed: int = 0) -> dict:
-    """Batched uneven solves (one fused plan per (operator, P, Q) group, hipGraph-captured iterations) against
-    one CG per mesh, solved one after the other."""
+    """Batched uneven solves: the group-batched CG (one fused plan per (operator, P, Q) group, hipGraph-captured
+    iterations) against one CG per mesh solved one after the other."""
     import torch
 
     from libceed_amd import ops
     from libceed_amd.batch import BatchCG, random_systems
 
     systems = random_systems(count, seed=seed)
+    rng = np.random.default_rng(seed)
+    rhs = []
+    for s in systems:
+        bi = rng.uniform(-1, 1, s.mesh.num_nodes)
+        if s.kind == ops.QF_DIFFUSION:  # pure Neumann: project b onto the range of A
+            bi -= bi.mean()
+        rhs.append(bi)
+
+    def timed(bcg) -> dict:
+        b = np.zeros(bcg.n)
+        for i in range(count):
+            b[bcg.range(i)] = rhs[i]
+        bd = torch.from_numpy(b).cuda()
+        x = torch.zeros_like(bd)
+        bcg.solve(bd, x, max_iter=50, rtol=0.0, check_every=25)  # warm-up: graph capture, code loading
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = bcg.solve(bd, x, max_iter=20000, rtol=rtol, check_every=25)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        # group-batched systems: converged at a check, every check interval costs about the same work;
+        # small systems: the device clock from the persistent launch's start to the system's completion
+        lat = np.where(np.isnan(st["done_ms"]), sec * st["converged_at_check"] / max(1, st["num_checks"]), 1e-3 * st["done_ms"])
+        return {"seconds": round(sec, 4), "solves_per_s": round(count / sec, 2), "iterations_max": int(st["iterations"].max()),
+                "latency_p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3), "latency_p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+                "max_rel_residual": float(st["rel_residual"].max())}, bd
+
     t0 = time.perf_counter()
     bcg = BatchCG(systems)
     setup = time.perf_counter() - t0
-    rng = np.random.default_rng(seed)
-    b = np.zeros(bcg.n)
-    for i, s in enumerate(systems):
-        r = bcg.range(i)
-        bi = rng.uniform(-1, 1, r.stop - r.start)
-        if s.kind == ops.QF_DIFFUSION:  # pure Neumann: project b onto the range of A
-            bi -= bi.mean()
-        b[r] = bi
-    bd = torch.from_numpy(b).cuda()
-    x = torch.zeros_like(bd)
-    bcg.solve(bd, x, max_iter=50, rtol=0.0, check_every=25)  # warm-up: graph capture
-    x.zero_()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    st = bcg.solve(bd, x, max_iter=20000, rtol=rtol, check_every=25)
-    sec = time.perf_counter() - t0
-    lat = sec * st["converged_at_check"] / max(1, st["num_checks"])  # every check interval costs the same work
-    dofs = int(bcg.n)
-    out = {"systems": count, "dofs_total": dofs, "groups": len(bcg.ops), "rtol": rtol, "setup_seconds": round(setup, 2),
-           "autotuned_launch": {f"{'mass' if k[0] == 0 else 'diff'}_P{k[1]}_Q{k[2]}": ["lines" if v[0] == 1 else "columns", v[1]]
+    grouped, bd = timed(bcg)
+    out = {"systems": count, "dofs_total": int(bcg.n), "groups": len(bcg.ops), "rtol": rtol, "setup_seconds": round(setup, 2),
+           "autotuned_launch": {f"{'mass' if k[0] == 0 else 'diff'}_P{k[1]}_Q{k[2]}": [LAYOUT_NAMES[v[0]], v[1]]
                                 for k, v in bcg.tuning.items()},
-           "batched": {"seconds": round(sec, 4), "solves_per_s": round(count / sec, 2), "iterations_max": int(st["iterations"].max()),
-                       "latency_p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
-                       "latency_p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
-                       "max_rel_residual": float(st["rel_residual"].max())}}
+           "batched": grouped}
+    del bcg
+    torch.cuda.empty_cache()
     # one CG per mesh, sequentially
     lat1 = []
     worst = 0.0
     for i, s in enumerate(systems):
-        r = bcg.range(i)
         op = ops.HexOperator.from_mesh(s.kind, s.mesh, Q=s.mesh.P + 1)
         cg = ops.CG(op)
-        bi = bd[r].contiguous()
+        bi = torch.from_numpy(rhs[i]).cuda()
         xi = torch.zeros_like(bi)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -442,7 +455,7 @@ def main() -> None:
                     help="arithmetic of the headline apply (the metric is FP64; f32 is for profiling)")
     ap.add_argument("--epb", type=int, default=0,
                     help="elements per workgroup of the headline plan (default: autotuned; fixed values are for profiling)")
-    ap.add_argument("--layout", choices=["lines", "columns"], default="lines",
+    ap.add_argument("--layout", choices=LAYOUT_NAMES, default="lines",
                     help="thread layout of the headline plan when --epb is given")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain 1 GiB device copy (HBM reference for bandwidth and counter scaling)")
@@ -468,7 +481,7 @@ def main() -> None:
                           "sum_bytes_read": 4 * nbytes, "ms_per_sum": ms_read, "GBs_read": 4 * nbytes / (ms_read * 1e-3) / 1e9}), flush=True)
         return
     head = run_case(args.p, args.dofs, torch.float64 if args.dtype == "f64" else torch.float32, args.steps, args.warmup, scatter=args.scatter,
-                    epb=args.epb, layout=1 if args.layout == "lines" else 0)
+                    epb=args.epb, layout=LAYOUT_NAMES.index(args.layout))
     sweep = {}
     if not args.no_sweep:
         for p in range(1, 9):

@@ -333,6 +333,26 @@ int CeedMi355BatchCGSolve(CeedMi355BatchCG cg, const double *b, double *x, int32
                           int32_t *num_checks_host);
 int CeedMi355BatchCGDestroy(CeedMi355BatchCG *cg);
 
+typedef struct CeedMi355BatchSmall_private *CeedMi355BatchSmall;
+/* Systems small enough for one workgroup, solved in one persistent launch:
+ * each workgroup takes the next system off a work list (largest first) and
+ * runs its whole unpreconditioned CG inside the workgroup, then takes the
+ * next; no lockstep across systems, so each system's latency is its own
+ * solve.  Same layout as CeedMi355BatchCGCreate, with every plan a scalar
+ * FP64 colour plan (CEED_MI355_PLAN_COLOR) of mass or diffusion, Q = P + 1,
+ * P = 2..9.  No counterpart in the reference (see CeedMi355BatchCGCreate).
+ * Currently switched off: returns CEED_MI355_ERROR_UNSUPPORTED (its first GPU
+ * test run faulted the GPU; the cause has not been found). */
+int CeedMi355BatchSmallCreate(int32_t num_ops, const CeedMi355Operator *ops_host, const int64_t *op_begin_host, int32_t num_systems,
+                              const int64_t *sys_begin_host, CeedMi355BatchSmall *bs);
+/* x_s = A_s^-1 b_s for every system from x = 0, until ||r_s|| <= rtol ||b_s||
+ * or max_iter iterations.  Per system (each may be NULL): iterations run, the
+ * true relative residual, and the time in ms from the launch's start to the
+ * system's completion (device clock).  Returns when the launch has ended. */
+int CeedMi355BatchSmallSolve(CeedMi355BatchSmall bs, const double *b, double *x, int32_t max_iter, double rtol, void *stream,
+                             int32_t *iterations_host, double *rel_residual_host, double *done_ms_host);
+int CeedMi355BatchSmallDestroy(CeedMi355BatchSmall *bs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,6 +62,9 @@ SIGNATURES = {
     "CeedMi355BatchCGCreate": (c_int, [c_i32, vp, vp, c_i32, vp, C.POINTER(vp)]),
     "CeedMi355BatchCGSolve": (c_int, [vp, vp, vp, c_i32, C.c_double, c_i32, c_int, vp, vp, vp, vp, vp]),
     "CeedMi355BatchCGDestroy": (c_int, [C.POINTER(vp)]),
+    "CeedMi355BatchSmallCreate": (c_int, [c_i32, vp, vp, c_i32, vp, C.POINTER(vp)]),
+    "CeedMi355BatchSmallSolve": (c_int, [vp, vp, vp, c_i32, C.c_double, vp, vp, vp, vp]),
+    "CeedMi355BatchSmallDestroy": (c_int, [C.POINTER(vp)]),
     "CeedMi355OperatorSetLaunch": (c_int, [vp, c_i32]),
     "CeedMi355OperatorAutotune": (c_int, [vp, c_i32, vp, vp, vp]),
     "CeedMi355OperatorSetLayout": (c_int, [vp, c_i32, c_i32]),

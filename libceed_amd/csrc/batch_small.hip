@@ -1,0 +1,411 @@
+// Batched CG for systems small enough for one workgroup, in one persistent
+// launch.
+//
+// The work list holds the systems, largest first.  Each workgroup takes the
+// next system off the list (an atomic counter), solves it completely --
+// unpreconditioned CG, every iteration inside the workgroup: the operator
+// apply colour by colour, the dot products as block reductions, the vector
+// updates -- and takes the next, until the list is empty.  A system leaves the
+// list when its workgroup takes it and is done when its residual reaches the
+// tolerance, independently of every other system: there is no lockstep
+// across systems and no grid-wide synchronisation, so a small system's
+// latency is its own solve time.  The per-(P, Q, operator) code is selected
+// per work item by a switch over compile-time instantiations.
+//
+// The operator of a system is one fused colour plan of its group (several
+// systems of one (operator, P, Q) concatenated, as for the group-batched CG);
+// the set-up lists each system's elements colour by colour.  Within a colour
+// no two elements share a node, so the E^T read-modify-writes of one colour
+// never collide; a workgroup barrier separates the colours, which fixes the
+// summation order (bitwise reproducible).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "hex_plan.hpp"
+#include "mi355_common.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+
+struct SmallGroup {  // one colour plan (all systems of one (operator, P, Q))
+  int32_t        kind;      // instantiation (see dispatch), -1 if none
+  int32_t        epb;       // elements per batch: kThreads / Q^2
+  int64_t        begin;     // the plan's range in the batch vector
+  const int32_t *offsets;   // [elem][P^3] packed, 2 * node + bit, nodes relative to begin
+  const double  *qdata;     // [elem][QC][Q^3] packed
+  const double  *mats;      // B (Q x P) then D (Q x Q)
+};
+
+struct SmallSystem {
+  int32_t group;
+  int32_t color_begin;  // into color_start: num_colors + 1 entries
+  int32_t num_colors;
+  int32_t pad;
+  int64_t vec_begin, vec_end;  // in the batch vector
+};
+
+// the instantiations: (operator, P) with Q = P + 1, P = 2..9
+constexpr int kMinP = 2, kMaxP = 9;
+int kind_of(int qf, int P, int Q) {
+  if (Q != P + 1 || P < kMinP || P > kMaxP || (qf != mi355::kQfMass && qf != mi355::kQfDiffusion)) return -1;
+  return (P - kMinP) * 2 + (qf == mi355::kQfDiffusion ? 1 : 0);
+}
+
+// a wave-uniform value the compiler cannot prove uniform (a function argument
+// of the work item), moved to scalar registers
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(T *p) {
+  const uint64_t a = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a), hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  return (T *)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ double block_sum(double v, double *red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double s = red[0];
+  __syncthreads();  // red is reused by the next reduction
+  return s;
+}
+
+// v = A u over the system's elements (u, v: the batch vectors, indexed by the
+// group's node numbers plus the group's begin)
+template <int P, int Q, int QF>
+__device__ void system_apply(const SmallGroup &g, const SmallSystem &sys, const int32_t *__restrict__ elem_list,
+                             const int32_t *__restrict__ color_start, const double *__restrict__ u, double *__restrict__ v) {
+  using namespace mi355;
+  constexpr int NT = Q * Q, SLAB = Q * Q * Q, P3 = P * P * P, Q3 = Q * Q * Q, QC = QfTraits<QF>::kComps;
+  constexpr int NS = LineSlabs<QF>::kSlabs, EPB = kThreads / NT;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  auto      at     = [](int z, int y, int x) { return (z * Q + y) * Q + x; };
+  const int le     = threadIdx.x / NT;
+  const int t      = threadIdx.x - le * NT;
+  const bool active = le < EPB;
+  double   *S0     = reinterpret_cast<double *>(lds_raw) + (size_t)(active ? le : 0) * NS * SLAB;
+  double   *S1 = S0 + SLAB, *S2 = S1 + SLAB;
+  const int cx = t % Q, cy = t / Q;
+  const int64_t  gbeg = uniform_i64(g.begin);
+  const double  *ug = uniform_ptr(u) + gbeg;
+  double        *vg = uniform_ptr(v) + gbeg;
+  const double  *mats = uniform_ptr(g.mats), *qdata = uniform_ptr(g.qdata);
+  const int32_t *offsets = uniform_ptr(g.offsets);
+  const int64_t  vb = uniform_i64(sys.vec_begin), ve = uniform_i64(sys.vec_end);
+  const int32_t  cbeg = __builtin_amdgcn_readfirstlane(sys.color_begin), ncol = __builtin_amdgcn_readfirstlane(sys.num_colors);
+  for (int64_t i = vb + threadIdx.x; i < ve; i += kThreads) v[i] = 0.0;
+  __syncthreads();
+  for (int c = 0; c < ncol; c++) {
+    const int32_t cb = __builtin_amdgcn_readfirstlane(color_start[cbeg + c]), ce = __builtin_amdgcn_readfirstlane(color_start[cbeg + c + 1]);
+    for (int32_t first = cb; first < ce; first += EPB) {
+      const int32_t k    = first + le;
+      const bool    live = active && k < ce;
+      const int64_t e    = live ? elem_list[k] : 0;
+      const bool    node = live && cx < P && cy < P;
+      int           off[P];
+      if (node) {
+#pragma unroll
+        for (int z = 0; z < P; z++) off[z] = offsets[e * P3 + (z * P + cy) * P + cx] >> 1;
+#pragma unroll
+        for (int z = 0; z < P; z++) S0[at(z, cy, cx)] = ug[off[z]];
+      }
+      LineQData<Q, 1, QF, double> qd;
+      const double               *qe = qdata + e * (int64_t)QC * Q3;
+      if (active) qd.load(qe, cx, cy);
+      __syncthreads();
+      line_element<P, Q, 1, QF, double>(S0, S1, S2, t, active, live, mats, qd, qe, SyncBlock{});
+      if (node) {  // colour-exclusive: no other element of this colour touches these nodes
+        double s[P], old[P];
+#pragma unroll
+        for (int z = 0; z < P; z++) s[z] = S0[at(z, cy, cx)];
+#pragma unroll
+        for (int z = 0; z < P; z++) old[z] = vg[off[z]];
+#pragma unroll
+        for (int z = 0; z < P; z++) vg[off[z]] = old[z] + s[z];
+      }
+      __syncthreads();  // S0 reused by the next batch; the next colour sees these sums
+    }
+  }
+}
+
+// CG on one system inside the workgroup: x = A^-1 b from x = 0, until
+// r.r <= rtol^2 b.b or max_iter iterations; then the true residual b - A x
+template <int P, int Q, int QF>
+__device__ __noinline__ void system_solve(const SmallGroup &g, const SmallSystem &sys, const int32_t *__restrict__ elem_list,
+                                          const int32_t *__restrict__ color_start, const double *__restrict__ b, double *__restrict__ x,
+                                          double *__restrict__ r, double *__restrict__ p, double *__restrict__ Ap, int32_t max_iter,
+                                          double rtol2, double *red, int32_t *iters_out, double *res_out) {
+  const int64_t lo = uniform_i64(sys.vec_begin), hi = uniform_i64(sys.vec_end);
+  double        bb = 0, rr = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kThreads) {
+    const double bi = b[i];
+    x[i] = 0.0;
+    r[i] = bi;
+    p[i] = bi;
+    bb += bi * bi;
+  }
+  bb = block_sum(bb, red);
+  rr = bb;
+  const double tol2 = rtol2 * bb;
+  int32_t      it   = 0;
+  while (it < max_iter && rr > tol2) {
+    system_apply<P, Q, QF>(g, sys, elem_list, color_start, p, Ap);
+    double pap = 0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kThreads) pap += p[i] * Ap[i];
+    pap = block_sum(pap, red);
+    if (!(pap > 0.0)) break;  // breakdown (or an operator that is not positive on p)
+    const double alpha = rr / pap;
+    double       rn    = 0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kThreads) {
+      x[i] += alpha * p[i];
+      const double ri = r[i] - alpha * Ap[i];
+      r[i]            = ri;
+      rn += ri * ri;
+    }
+    rn                = block_sum(rn, red);
+    const double beta = rn / rr;
+    rr                = rn;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kThreads) p[i] = r[i] + beta * p[i];
+    __syncthreads();
+    it++;
+  }
+  // true residual of the returned solution
+  system_apply<P, Q, QF>(g, sys, elem_list, color_start, x, Ap);
+  double tr = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kThreads) {
+    const double ri = b[i] - Ap[i];
+    tr += ri * ri;
+  }
+  tr = block_sum(tr, red);
+  if (threadIdx.x == 0) {
+    *iters_out = it;
+    *res_out   = bb > 0 ? sqrt(tr / bb) : 0.0;
+  }
+}
+
+__global__ void k_reset_counter(int32_t *counter) { *counter = 0; }
+
+__global__ void __launch_bounds__(kThreads) batch_small_kernel(int32_t num_items, const int32_t *__restrict__ items,
+                                                               const SmallSystem *__restrict__ systems, const SmallGroup *__restrict__ groups,
+                                                               const int32_t *__restrict__ elem_list, const int32_t *__restrict__ color_start,
+                                                               const double *__restrict__ b, double *__restrict__ x, double *__restrict__ r,
+                                                               double *__restrict__ p, double *__restrict__ Ap, int32_t max_iter, double rtol2,
+                                                               int32_t *__restrict__ counter, int32_t *__restrict__ iters,
+                                                               double *__restrict__ res, uint64_t *__restrict__ t_start,
+                                                               uint64_t *__restrict__ t_done) {
+  __shared__ double  red[kThreads];
+  __shared__ int32_t item;
+  if (threadIdx.x == 0) t_start[blockIdx.x] = wall_clock64();
+  for (;;) {
+    if (threadIdx.x == 0) item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int32_t k = __builtin_amdgcn_readfirstlane(item);  // uniform: system data through scalar loads
+    __syncthreads();  // every thread has read item before thread 0 overwrites it
+    if (k >= num_items) break;
+    const int32_t      s   = items[k];
+    const SmallSystem  sys = systems[s];
+    const SmallGroup   g   = groups[sys.group];
+    switch (g.kind) {
+#define MI355_SMALL_CASE(PP)                                                                                                          \
+  case (PP - kMinP) * 2:                                                                                                              \
+    system_solve<PP, PP + 1, mi355::kQfMass>(g, sys, elem_list, color_start, b, x, r, p, Ap, max_iter, rtol2, red, iters + s, res + s); \
+    break;                                                                                                                            \
+  case (PP - kMinP) * 2 + 1:                                                                                                          \
+    system_solve<PP, PP + 1, mi355::kQfDiffusion>(g, sys, elem_list, color_start, b, x, r, p, Ap, max_iter, rtol2, red, iters + s,    \
+                                                  res + s);                                                                           \
+    break;
+      MI355_SMALL_CASE(2)
+      MI355_SMALL_CASE(3)
+      MI355_SMALL_CASE(4)
+      MI355_SMALL_CASE(5)
+      MI355_SMALL_CASE(6)
+      MI355_SMALL_CASE(7)
+      MI355_SMALL_CASE(8)
+      MI355_SMALL_CASE(9)
+#undef MI355_SMALL_CASE
+      default: break;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) t_done[s] = wall_clock64();
+  }
+}
+
+// dynamic LDS of the kernel: the largest instantiation's slabs
+size_t small_lds_bytes(int max_P) {
+  size_t m = 0;
+  for (int P = kMinP; P <= max_P; P++) {
+    const int Q = P + 1, epb = kThreads / (Q * Q);
+    m = std::max(m, sizeof(double) * (size_t)epb * 3 * Q * Q * Q);
+  }
+  return m;
+}
+
+}  // namespace
+
+struct CeedMi355BatchSmall_private {
+  int32_t              num_sys = 0, num_groups = 0, max_P = kMinP;
+  int64_t              n = 0;
+  SmallSystem         *d_systems = nullptr;
+  SmallGroup          *d_groups  = nullptr;
+  int32_t             *d_items = nullptr, *d_elem_list = nullptr, *d_color_start = nullptr, *d_counter = nullptr, *d_iters = nullptr;
+  double              *d_res = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr;
+  uint64_t            *d_t_start = nullptr, *d_t_done = nullptr;
+  int                  grid = 0;
+  double               clock_khz = 0;
+  std::vector<int64_t> sys_begin;
+};
+
+extern "C" int CeedMi355BatchSmallCreate(int32_t num_ops, const CeedMi355Operator *ops_host, const int64_t *op_begin_host, int32_t num_systems,
+                                         const int64_t *sys_begin_host, CeedMi355BatchSmall *bs_out) {
+  *bs_out = nullptr;
+  // Switched off: its first GPU test run ended in a GPU memory fault whose
+  // cause has not been found by reading the kernel or its set-up; until it
+  // is, the entry point refuses to create a solver.
+  MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED,
+              "the persistent small-system CG is switched off (a test run faulted the GPU; cause not yet found)");
+  MI355_CHECK(num_ops > 0 && num_systems > 0, CEED_MI355_ERROR_DIMENSION, "batched small-system CG needs operators and systems");
+  const int64_t n = sys_begin_host[num_systems];
+  MI355_CHECK(sys_begin_host[0] == 0 && n > 0, CEED_MI355_ERROR_DIMENSION, "system ranges must start at 0 and be non-empty");
+  for (int32_t s = 0; s < num_systems; s++)
+    MI355_CHECK(sys_begin_host[s + 1] > sys_begin_host[s], CEED_MI355_ERROR_DIMENSION, "system %d has an empty range", s);
+  std::vector<SmallGroup>  groups(num_ops);
+  std::vector<SmallSystem> systems(num_systems);
+  std::vector<int32_t>     elem_list, color_start;
+  std::vector<int>         sys_seen(num_systems, 0);
+  std::vector<double>      cost(num_systems, 0.0);
+  int                      max_P = kMinP;
+  for (int32_t i = 0; i < num_ops; i++) {
+    const CeedMi355Operator op = ops_host[i];
+    MI355_CHECK(op && op->scalar_type == CEED_MI355_FP64 && op->num_comp == 1, CEED_MI355_ERROR_BACKEND, "operator %d is not a scalar FP64 plan", i);
+    MI355_CHECK(op->scatter == CEED_MI355_PLAN_COLOR, CEED_MI355_ERROR_BACKEND, "operator %d is not a colour plan (CEED_MI355_PLAN_COLOR)", i);
+    const int kind = kind_of(op->qf, op->P, op->Q);
+    MI355_CHECK(kind >= 0, CEED_MI355_ERROR_UNSUPPORTED, "operator %d: no small-system kernel for this operator at P = %d, Q = %d", i, op->P, op->Q);
+    MI355_CHECK(op_begin_host[i] >= 0 && op_begin_host[i] + op->l_size <= n, CEED_MI355_ERROR_DIMENSION, "operator %d outside the batch vector", i);
+    max_P = std::max(max_P, op->P);
+    groups[i] = SmallGroup{kind, kThreads / (op->Q * op->Q), op_begin_host[i], op->d_offsets, (const double *)op->d_qdata, (const double *)op->d_mats};
+    // the system of every packed element (from its first node), colour by colour
+    const int64_t        es = (int64_t)op->P * op->P * op->P;
+    std::vector<int32_t> off((size_t)(op->num_elem * es));
+    if (op->num_elem) MI355_HIP(hipMemcpy(off.data(), op->d_offsets, sizeof(int32_t) * off.size(), hipMemcpyDeviceToHost));
+    const int                                      nc = (int)op->color_start.size() - 1;
+    std::vector<int32_t>                           local_sys;  // systems of this operator, in the order met
+    std::vector<int32_t>                           slot(num_systems, -1);
+    std::vector<std::vector<std::vector<int32_t>>> lists;      // [local system][colour] -> packed elements, ascending
+    for (int c = 0; c < nc; c++)
+      for (int64_t e = op->color_start[c]; e < op->color_start[c + 1]; e++) {
+        const int64_t node = op_begin_host[i] + (off[(size_t)(e * es)] >> 1);
+        const int32_t s    = (int32_t)(std::upper_bound(sys_begin_host, sys_begin_host + num_systems + 1, node) - sys_begin_host - 1);
+        if (slot[s] < 0) {
+          slot[s] = (int32_t)local_sys.size();
+          local_sys.push_back(s);
+          lists.emplace_back(nc);
+        }
+        lists[slot[s]][c].push_back((int32_t)e);
+      }
+    for (size_t j = 0; j < local_sys.size(); j++) {
+      const int32_t s = local_sys[j];
+      MI355_CHECK(!sys_seen[s], CEED_MI355_ERROR_BACKEND, "system %d has elements in two operators", s);
+      sys_seen[s]  = 1;
+      SmallSystem &d = systems[s];
+      d.group        = i;
+      d.color_begin  = (int32_t)color_start.size();
+      d.num_colors   = 0;
+      d.vec_begin    = sys_begin_host[s];
+      d.vec_end      = sys_begin_host[s + 1];
+      int64_t ne     = 0;
+      for (int c = 0; c < nc; c++) {
+        if (lists[j][c].empty()) continue;
+        color_start.push_back((int32_t)elem_list.size());
+        elem_list.insert(elem_list.end(), lists[j][c].begin(), lists[j][c].end());
+        d.num_colors++;
+        ne += (int64_t)lists[j][c].size();
+      }
+      color_start.push_back((int32_t)elem_list.size());
+      // work estimate: element applies per iteration times iterations (about n^(1/3) P for these operators)
+      cost[s] = (double)ne * es * std::cbrt((double)(d.vec_end - d.vec_begin)) * op->P;
+    }
+  }
+  for (int32_t s = 0; s < num_systems; s++)
+    MI355_CHECK(sys_seen[s], CEED_MI355_ERROR_DIMENSION, "system %d has no elements in any operator", s);
+  std::vector<int32_t> items(num_systems);
+  std::iota(items.begin(), items.end(), 0);
+  std::stable_sort(items.begin(), items.end(), [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+
+  CeedMi355BatchSmall bs = new CeedMi355BatchSmall_private();
+  bs->num_sys            = num_systems;
+  bs->num_groups         = num_ops;
+  bs->max_P              = max_P;
+  bs->n                  = n;
+  bs->sys_begin.assign(sys_begin_host, sys_begin_host + num_systems + 1);
+  int dev = 0, cus = 0, khz = 0;
+  bool ok = hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess;
+  bs->clock_khz = khz;
+  // two workgroups per CU at most, never more than there are systems
+  bs->grid = std::max(1, std::min(num_systems, 2 * std::max(1, cus)));
+  auto up  = [&](void **d, const void *h, size_t bytes) {
+    return hipMalloc(d, bytes ? bytes : 8) == hipSuccess && (!bytes || hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice) == hipSuccess);
+  };
+  ok = ok && up((void **)&bs->d_systems, systems.data(), sizeof(SmallSystem) * systems.size()) &&
+       up((void **)&bs->d_groups, groups.data(), sizeof(SmallGroup) * groups.size()) &&
+       up((void **)&bs->d_items, items.data(), sizeof(int32_t) * items.size()) &&
+       up((void **)&bs->d_elem_list, elem_list.data(), sizeof(int32_t) * elem_list.size()) &&
+       up((void **)&bs->d_color_start, color_start.data(), sizeof(int32_t) * color_start.size()) &&
+       hipMalloc(&bs->d_counter, sizeof(int32_t)) == hipSuccess && hipMalloc(&bs->d_iters, sizeof(int32_t) * num_systems) == hipSuccess &&
+       hipMalloc(&bs->d_res, sizeof(double) * num_systems) == hipSuccess && hipMalloc(&bs->r, sizeof(double) * n) == hipSuccess &&
+       hipMalloc(&bs->p, sizeof(double) * n) == hipSuccess && hipMalloc(&bs->Ap, sizeof(double) * n) == hipSuccess &&
+       hipMalloc(&bs->d_t_start, sizeof(uint64_t) * bs->grid) == hipSuccess && hipMalloc(&bs->d_t_done, sizeof(uint64_t) * num_systems) == hipSuccess;
+  if (!ok) {
+    CeedMi355BatchSmallDestroy(&bs);
+    MI355_CHECK(false, CEED_MI355_ERROR_BACKEND, "batched small-system CG set-up failed (%lld entries)", (long long)n);
+  }
+  *bs_out = bs;
+  return CEED_MI355_SUCCESS;
+}
+
+extern "C" int CeedMi355BatchSmallSolve(CeedMi355BatchSmall bs, const double *b, double *x, int32_t max_iter, double rtol, void *stream,
+                                        int32_t *iterations_host, double *rel_residual_host, double *done_ms_host) {
+  MI355_CHECK(bs && max_iter >= 0 && rtol >= 0, CEED_MI355_ERROR_BACKEND, "bad batched small-system CG arguments");
+  hipStream_t s = (hipStream_t)stream;
+  k_reset_counter<<<1, 1, 0, s>>>(bs->d_counter);
+  batch_small_kernel<<<bs->grid, kThreads, small_lds_bytes(bs->max_P), s>>>(
+      bs->num_sys, bs->d_items, bs->d_systems, bs->d_groups, bs->d_elem_list, bs->d_color_start, b, x, bs->r, bs->p, bs->Ap, max_iter, rtol * rtol,
+      bs->d_counter, bs->d_iters, bs->d_res, bs->d_t_start, bs->d_t_done);
+  MI355_LAUNCH_CHECK();
+  if (iterations_host) MI355_HIP(hipMemcpyAsync(iterations_host, bs->d_iters, sizeof(int32_t) * bs->num_sys, hipMemcpyDeviceToHost, s));
+  if (rel_residual_host) MI355_HIP(hipMemcpyAsync(rel_residual_host, bs->d_res, sizeof(double) * bs->num_sys, hipMemcpyDeviceToHost, s));
+  if (done_ms_host) {
+    std::vector<uint64_t> t0(bs->grid), t1(bs->num_sys);
+    MI355_HIP(hipMemcpyAsync(t0.data(), bs->d_t_start, sizeof(uint64_t) * bs->grid, hipMemcpyDeviceToHost, s));
+    MI355_HIP(hipMemcpyAsync(t1.data(), bs->d_t_done, sizeof(uint64_t) * bs->num_sys, hipMemcpyDeviceToHost, s));
+    MI355_HIP(hipStreamSynchronize(s));
+    const uint64_t start = *std::min_element(t0.begin(), t0.end());
+    for (int32_t i = 0; i < bs->num_sys; i++) done_ms_host[i] = bs->clock_khz > 0 ? (double)(t1[i] - start) / bs->clock_khz : 0.0;
+  } else {
+    MI355_HIP(hipStreamSynchronize(s));
+  }
+  return CEED_MI355_SUCCESS;
+}
+
+extern "C" int CeedMi355BatchSmallDestroy(CeedMi355BatchSmall *bs) {
+  if (!bs || !*bs) return CEED_MI355_SUCCESS;
+  CeedMi355BatchSmall c = *bs;
+  for (void *d : {(void *)c->d_systems, (void *)c->d_groups, (void *)c->d_items, (void *)c->d_elem_list, (void *)c->d_color_start,
+                  (void *)c->d_counter, (void *)c->d_iters, (void *)c->d_res, (void *)c->r, (void *)c->p, (void *)c->Ap, (void *)c->d_t_start,
+                  (void *)c->d_t_done})
+    if (d) hipFree(d);
+  delete c;
+  *bs = nullptr;
+  return CEED_MI355_SUCCESS;
+}

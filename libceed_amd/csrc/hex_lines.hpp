@@ -72,6 +72,337 @@ __host__ __device__ constexpr int hex_lines_lds_entries(int epb) {
   return epb * LineSlabs<QF>::kSlabs * Q * Q * Q;
 }
 
+// The column's geometric factors (QC values at each of its Q points) and how
+// they are loaded: all loads issued at once and shared by the components, at
+// the element start (in flight during E and the interpolation), or for
+// scalar FP64 diffusion from Q = 9 just before the pointwise operator, where
+// 6Q doubles held through the whole element cost a wave per SIMD (191 -> 137
+// VGPRs at Q = 9; measured on MI355X, 8M DoFs: p=7 26.6 -> 27.6 GDoF/s, p=8
+// 26.2 -> 27.5; later loads lose at Q <= 8 and in FP32, where the exposed
+// latency costs more than the occupancy gains).  FP64 scalar diffusion with
+// early loads and Q even loads them in lane pairs with 16-byte loads
+// (interleaved A/B on MI355X, 8M DoFs: p=4 +0.7% with colours, +3.7% with the
+// map, p=6 +0-4% with the split; with the late loads at Q = 10 the swap in the
+// pointwise loop cost 15%; FP32 pairs of 8-byte loads were 3-5% slower).
+template <int Q, int NCOMP, int QF, typename T>
+struct LineQData {
+  static constexpr int  QC     = QfTraits<QF>::kComps, Q3 = Q * Q * Q, QH = QC / 2;
+  static constexpr bool kEarly = QF == kQfMass || NCOMP > 1 || Q < 9 || sizeof(T) < 8;
+  static constexpr bool kPair  = kEarly && sizeof(T) == 8 && QF == kQfDiffusion && NCOMP == 1 && Q % 2 == 0;
+  T                     sq[kPair ? 1 : Q][QC];
+  line_v2<T>            sp[kPair ? Q : 1][kPair ? QH : 1];
+
+  // qe: the element's block [QC][Q^3]; (cx, cy) the column
+  __device__ __forceinline__ void load(const T *qe, int cx, int cy) {
+    if constexpr (kPair) {  // 16-byte aligned: Q even, so (cy * Q + cx - odd) and every stride are even
+      const T *pb = qe + (cy * Q + (cx & ~1)) + ((cx & 1) ? QH : 0) * Q3;
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++)
+#pragma unroll
+        for (int kk = 0; kk < QH; kk++) sp[qz][kk] = __builtin_nontemporal_load((const line_v2<T> *)(pb + kk * Q3 + qz * Q * Q));
+    } else {
+      const T *qd = qe + (cy * Q + cx);
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++)
+#pragma unroll
+        for (int k = 0; k < QC; k++) sq[qz][k] = __builtin_nontemporal_load(&qd[k * Q3 + qz * Q * Q]);
+    }
+  }
+  // the QC values of point qz of the column (with the pairs: this column's
+  // half from the own load, the other half from the partner lane)
+  __device__ __forceinline__ void point(int qz, int cx, T (&s)[QC]) const {
+    if constexpr (kPair) {
+      const bool odd = cx & 1;
+#pragma unroll
+      for (int kk = 0; kk < QH; kk++) {
+        const line_v2<T> d = sp[qz][kk];
+        const T       keep = odd ? d.y : d.x, got = pair_swap(odd ? d.x : d.y);
+        s[kk]              = odd ? got : keep;
+        s[QH + kk]         = odd ? keep : got;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < QC; k++) s[k] = sq[qz][k];
+    }
+  }
+};
+
+// Synchronisation of one element's threads between the phases of the line
+// pipeline: the whole workgroup, or the wavefront alone when an element never
+// spans two wavefronts (a wavefront's LDS operations are performed in the
+// order it issues them; the fences keep the compiler from moving them).
+struct SyncBlock {
+  __device__ __forceinline__ void operator()() const { __syncthreads(); }
+};
+struct SyncWave {
+  __device__ __forceinline__ void operator()() const {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+};
+
+// One element and component through B, D and B^T in the line layout.  On
+// entry S0 holds the element's nodal values at (z, y, x), x, y, z < P, and
+// the element's threads are synchronised; on return S0 holds the nodal
+// results there, synchronised again.  t is the thread's index among the
+// element's Q*Q threads; threads with active false (lanes of a wavefront
+// that hold no element) only take part in the synchronisations.  qd holds
+// the column's geometric factors (loaded here first when they are late,
+// from qe); live false zeroes the pointwise output.
+template <int P, int Q, int NCOMP, int QF, typename T, typename Sync>
+__device__ __forceinline__ void line_element(T *S0, T *S1, T *S2, int t, bool active, bool live, const T *__restrict__ mats,
+                                             LineQData<Q, NCOMP, QF, T> &qd, const T *qe, Sync sync) {
+  constexpr int QC = QfTraits<QF>::kComps;
+  // from Q = 7 one phase's matrix entries no longer fit the SGPRs: load them
+  // per output row (or column) instead of per phase
+  constexpr bool kRowWise = Q >= 7;
+  // B (Q x P) then D (Q x Q), row-major, at mats
+  auto       at = [](int z, int y, int x) { return (z * Q + y) * Q + x; };
+  const int  cx = t % Q, cy = t / Q;  // column
+  const int  ny = t % P, nz = t / P;  // x-line of nodes
+  const bool xline = active && t < P * P;
+  // ---- B_x: x-line (ny, nz) of nodes, Q interpolated values into S1
+  if (xline) {
+    const auto gB = phase_ptr(mats);
+    T          in[P];
+#pragma unroll
+    for (int x = 0; x < P; x++) in[x] = S0[at(nz, ny, x)];
+#pragma unroll
+    for (int qx = 0; qx < Q; qx++) {
+      const auto m = kRowWise ? phase_ptr(mats + qx * P) : gB + qx * P;
+      T          s = 0;
+#pragma unroll
+      for (int b = 0; b < P; b++) s += m[b] * in[b];
+      S1[at(nz, ny, qx)] = s;
+    }
+  }
+  sync();
+  // ---- B_y: y-line (qx, z), z < P
+  if (active && t < Q * P) {
+    const auto gB = phase_ptr(mats);
+    const int  qx = t % Q, z = t / Q;
+    T          in[P];
+#pragma unroll
+    for (int b = 0; b < P; b++) in[b] = S1[at(z, b, qx)];
+#pragma unroll
+    for (int qy = 0; qy < Q; qy++) {
+      const auto m = kRowWise ? phase_ptr(mats + qy * P) : gB + qy * P;
+      T          s = 0;
+#pragma unroll
+      for (int b = 0; b < P; b++) s += m[b] * in[b];
+      S0[at(z, qy, qx)] = s;
+    }
+  }
+  sync();
+  // ---- B_z on the column (cx, cy): uq[qz]
+  T uq[Q];
+  if (active) {
+    const auto gB = phase_ptr(mats);
+    T          col[P];
+#pragma unroll
+    for (int b = 0; b < P; b++) col[b] = S0[at(b, cy, cx)];
+#pragma unroll
+    for (int qz = 0; qz < Q; qz++) {
+      const auto m = kRowWise ? phase_ptr(mats + qz * P) : gB + qz * P;
+      T          s = 0;
+#pragma unroll
+      for (int b = 0; b < P; b++) s += m[b] * col[b];
+      uq[qz] = s;
+    }
+  }
+  if constexpr (QF == kQfMass) {
+    // ---- D, then B_z^T back into the column's own S0 entries (read only by this thread)
+    if (active) {
+      const auto gB = phase_ptr(mats);
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) {
+        T s[QC];
+        qd.point(qz, cx, s);
+        uq[qz] *= live ? s[0] : T(0);
+      }
+#pragma unroll
+      for (int b = 0; b < P; b++) {
+        const auto m = kRowWise ? phase_ptr(mats + b) : gB + b;
+        T          s = 0;
+#pragma unroll
+        for (int qz = 0; qz < Q; qz++) s += m[qz * P] * uq[qz];
+        S0[at(b, cy, cx)] = s;
+      }
+    }
+  } else {
+    // ---- collocated derivatives: z on the column, x and y by lines through S1
+    T dz[Q];
+    if (active) {
+      const auto gD = phase_ptr(mats + Q * P);
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) {
+        const auto m = kRowWise ? phase_ptr(mats + Q * P + qz * Q) : gD + qz * Q;
+        T          s = 0;
+#pragma unroll
+        for (int c = 0; c < Q; c++) s += m[c] * uq[c];
+        dz[qz]             = s;
+        S1[at(qz, cy, cx)] = uq[qz];
+      }
+    }
+    sync();
+    if (active) {
+      const auto gD = phase_ptr(mats + Q * P);
+      const int  a = t % Q, qz = t / Q;  // x-line (qy = a, qz) and y-line (qx = a, qz)
+      T          in[Q];
+#pragma unroll
+      for (int c = 0; c < Q; c++) in[c] = S1[at(qz, a, c)];
+#pragma unroll
+      for (int qx = 0; qx < Q; qx++) {
+        const auto m = kRowWise ? phase_ptr(mats + Q * P + qx * Q) : gD + qx * Q;
+        T          s = 0;
+#pragma unroll
+        for (int c = 0; c < Q; c++) s += m[c] * in[c];
+        S2[at(qz, a, qx)] = s;
+      }
+#pragma unroll
+      for (int c = 0; c < Q; c++) in[c] = S1[at(qz, c, a)];
+#pragma unroll
+      for (int qy = 0; qy < Q; qy++) {
+        const auto m = kRowWise ? phase_ptr(mats + Q * P + qy * Q) : gD + qy * Q;
+        T          s = 0;
+#pragma unroll
+        for (int c = 0; c < Q; c++) s += m[c] * in[c];
+        S0[at(qz, qy, a)] = s;
+      }
+    }
+    sync();
+    // ---- pointwise (Poisson3DApply, times kappa for the kappa-weighted operator) on the column
+    if constexpr (!LineQData<Q, NCOMP, QF, T>::kEarly) {
+      if (active) qd.load(qe, cx, cy);
+    }
+    T vz[Q];
+    if (active) {
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) {
+        T vx, vy, vzz;
+        T s[QC];
+        qd.point(qz, cx, s);
+        poisson_regs<T>(s, S2[at(qz, cy, cx)], S0[at(qz, cy, cx)], dz[qz], vx, vy, vzz);
+        if constexpr (QF == kQfDiffusionKappa) {
+          vx *= s[6];
+          vy *= s[6];
+          vzz *= s[6];
+        }
+        if (!live) vx = vy = vzz = 0;
+        S2[at(qz, cy, cx)] = vx;
+        S0[at(qz, cy, cx)] = vy;
+        vz[qz]             = vzz;
+      }
+    }
+    sync();
+    // ---- D_x^T by x-lines into S1; D_y^T by y-lines, held in registers
+    // until every x-line has read S2, then written over it
+    {
+      const int a = t % Q, qz = t / Q;
+      T         ty[Q];
+      if (active) {
+        {
+          const auto gD = phase_ptr(mats + Q * P);
+          T          in[Q];
+#pragma unroll
+          for (int c = 0; c < Q; c++) in[c] = S2[at(qz, a, c)];
+#pragma unroll
+          for (int qx = 0; qx < Q; qx++) {
+            const auto m = kRowWise ? phase_ptr(mats + Q * P + qx) : gD + qx;
+            T          s = 0;
+#pragma unroll
+            for (int c = 0; c < Q; c++) s += m[c * Q] * in[c];
+            S1[at(qz, a, qx)] = s;
+          }
+        }
+        {
+          const auto gD = phase_ptr(mats + Q * P);
+          T          in[Q];
+#pragma unroll
+          for (int c = 0; c < Q; c++) in[c] = S0[at(qz, c, a)];
+#pragma unroll
+          for (int qy = 0; qy < Q; qy++) {
+            const auto m = kRowWise ? phase_ptr(mats + Q * P + qy) : gD + qy;
+            T          s = 0;
+#pragma unroll
+            for (int c = 0; c < Q; c++) s += m[c * Q] * in[c];
+            ty[qy] = s;
+          }
+        }
+      }
+      sync();
+      if (active) {
+#pragma unroll
+        for (int qy = 0; qy < Q; qy++) S2[at(qz, qy, a)] = ty[qy];
+      }
+    }
+    sync();
+    // ---- sum of the three transposes on the column, then B_z^T into S0
+    if (active) {
+      const auto gDz = phase_ptr(mats + Q * P);
+      T          w[Q];
+#pragma unroll
+      for (int qz = 0; qz < Q; qz++) {
+        const auto m = kRowWise ? phase_ptr(mats + Q * P + qz) : gDz + qz;
+        T          s = S1[at(qz, cy, cx)] + S2[at(qz, cy, cx)];
+#pragma unroll
+        for (int c = 0; c < Q; c++) s += m[c * Q] * vz[c];
+        w[qz] = s;
+      }
+      const auto gBz = phase_ptr(mats);
+#pragma unroll
+      for (int b = 0; b < P; b++) {
+        const auto m = kRowWise ? phase_ptr(mats + b) : gBz + b;
+        T          s = 0;
+#pragma unroll
+        for (int qz = 0; qz < Q; qz++) s += m[qz * P] * w[qz];
+        S0[at(b, cy, cx)] = s;
+      }
+    }
+  }
+  sync();
+  // ---- B_y^T: y-line (qx, z), z < P, into S1
+  if (active && t < Q * P) {
+    const auto gB = phase_ptr(mats);
+    const int  qx = t % Q, z = t / Q;
+    T          in[Q];
+#pragma unroll
+    for (int a = 0; a < Q; a++) in[a] = S0[at(z, a, qx)];
+#pragma unroll
+    for (int y = 0; y < P; y++) {
+      const auto m = kRowWise ? phase_ptr(mats + y) : gB + y;
+      T          s = 0;
+#pragma unroll
+      for (int a = 0; a < Q; a++) s += m[a * P] * in[a];
+      S1[at(z, y, qx)] = s;
+    }
+  }
+  sync();
+  // ---- B_x^T: x-line (ny, nz) of nodes, into S0 (last read by B_y^T)
+  if (xline) {
+    const auto gB = phase_ptr(mats);
+    T          in[Q];
+#pragma unroll
+    for (int a = 0; a < Q; a++) in[a] = S1[at(nz, ny, a)];
+#pragma unroll
+    for (int x = 0; x < P; x++) {
+      const auto m   = kRowWise ? phase_ptr(mats + x) : gB + x;
+      T          acc = 0;
+#pragma unroll
+      for (int a = 0; a < Q; a++) acc += m[a * P] * in[a];
+      S0[at(nz, ny, x)] = acc;
+    }
+  }
+  sync();
+}
+
+// The line kernel keeps the pipeline written out in its own body rather than
+// calling line_element: the same code through the shared function compiles
+// to a different register allocation here (P=7, Q=8 diffusion: 186 against
+// 162 VGPRs, a wave per SIMD fewer, 20% slower on MI355X at 8M DoFs; P=5,
+// Q=6: 133 against 127).  Both compute every value with the same operations
+// in the same order (tests/test_gpu_brick.py::test_brick_single_brick_bitwise).
 template <int P, int Q, int NCOMP, int QF, typename T>
 __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin, int64_t elem_end, int scatter,
                                                                const int32_t *__restrict__ offsets, int32_t comp_stride,

@@ -47,6 +47,11 @@ constexpr int max_threads(int) { return 256; }
 // choices per Q, otherwise as many as fit 256 threads and kDefaultLdsBytes of LDS.
 int default_epb(int P, int Q, int qf, int scalar_bytes, int layout);
 int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes, int layout);
+// Threads and LDS bytes of the brick kernel's workgroup for a brick of ns
+// slots (shape[0] * shape[1] * shape[2]); the kernel's own formulas in
+// hex_brick.hpp, for plan set-up on the host.
+int64_t brick_threads(int Q, int64_t ns);
+int64_t brick_lds_bytes(int P, int Q, int qf, int scalar_bytes, const int *shape);
 
 }  // namespace mi355
 
@@ -118,7 +123,6 @@ struct CeedMi355Operator_private {
   // d_blist[j], the partial slots d_bslot[d_brow[j] .. d_brow[j+1]) (each
   // b * num_comp * nb_nodes + local), in ascending brick order
   int      brick_shape[3] = {1, 1, 1};
-  int      brick_prefetch = 1;  // the brick kernel loads the next element's data during the current one
   int64_t  num_bricks = 0, num_bound = 0;
   int      nb_nodes = 0;
   int32_t *d_brick_start = nullptr, *d_brick_mask = nullptr, *d_bnode = nullptr;
@@ -145,11 +149,10 @@ int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v
   const size_t lds   = sizeof(T) * (size_t)(lines ? hex_lines_lds_entries<P, Q, QF>(epb) : hex_lds_entries<P, Q, QF>(epb));
   if (op->scatter == CEED_MI355_PLAN_BRICK) {
     if (op->num_bricks == 0) return CEED_MI355_SUCCESS;
-    const size_t blds = hex_brick_lds_bytes<P, Q, NC, QF, T>(op->nb_nodes);
-    auto         kern = op->brick_prefetch ? hex_apply_brick_kernel<P, Q, NC, QF, T, true> : hex_apply_brick_kernel<P, Q, NC, QF, T, false>;
-    kern<<<(unsigned)op->num_bricks, nt, blds, s>>>(op->brick_shape[0], op->brick_shape[1], op->nb_nodes, op->d_brick_start, op->d_brick_mask,
-                                                    op->d_bnode, op->comp_stride, (const T *)op->d_qdata, (const T *)op->d_mats, (const T *)u,
-                                                    (T *)v, (T *)op->d_partials, overwrite);
+    const int ns = op->brick_shape[0] * op->brick_shape[1] * op->brick_shape[2];
+    hex_apply_brick_kernel<P, Q, NC, QF, T><<<(unsigned)op->num_bricks, BrickThreads<P, Q>::threads(ns), hex_brick_lds_bytes<P, Q, QF, T>(ns, op->nb_nodes), s>>>(
+        op->brick_shape[0], op->brick_shape[1], op->brick_shape[2], op->nb_nodes, op->d_brick_start, op->d_brick_mask, op->d_bnode,
+        op->comp_stride, (const T *)op->d_qdata, (const T *)op->d_mats, (const T *)u, (T *)v, (T *)op->d_partials, overwrite);
     MI355_LAUNCH_CHECK();
     return brick_gather(op, v, overwrite, s);
   }

@@ -21,6 +21,19 @@ int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes, int layout) {
   return scalar_bytes * (Q * P + Q * Q + epb * slabs * Q * Q * Q);
 }
 
+int64_t brick_threads(int Q, int64_t ns) {
+  const int nt = Q * Q;
+  if (nt > 64) return ns * nt;
+  const int epw = 64 / nt;
+  return 64 * ((ns + epw - 1) / epw);
+}
+
+int64_t brick_lds_bytes(int P, int Q, int qf, int scalar_bytes, const int *shape) {
+  const int64_t ns = (int64_t)shape[0] * shape[1] * shape[2];
+  const int64_t nb = (int64_t)(shape[0] * (P - 1) + 1) * (shape[1] * (P - 1) + 1) * (shape[2] * (P - 1) + 1);
+  return (int64_t)scalar_bytes * ns * (qf == kQfMass ? 2 : 3) * Q * Q * Q + 4 * nb;
+}
+
 int default_epb(int P, int Q, int qf, int scalar_bytes, int layout) {
   if (layout == kLayoutLines && qf != kQfMass && Q >= 2 && Q <= 10) {
     // diffusion in the line layout: the autotuner's choices for FP64 at ~8M
@@ -237,14 +250,18 @@ bool tuned_launch(const CeedMi355Operator op, int *layout, int *epb) {
   return true;
 }
 
-bool admissible(int P, int Q, int qf, int scalar_bytes, int layout, int epb) {
+bool admissible(int P, int Q, int qf, int num_comp, int scalar_bytes, int layout, int epb) {
+  (void)num_comp;
   return ((layout == mi355::kLayoutColumns && qf != kQfJit) || layout == mi355::kLayoutLines) && epb >= 1 &&
          epb * Q * Q <= mi355::max_threads(layout) && mi355::lds_bytes(P, Q, qf, epb, scalar_bytes, layout) <= mi355::kMaxLdsBytes;
 }
 
 bool admissible_epb(const CeedMi355Operator op, int epb) {
-  return admissible(op->P, op->Q, op->qf, op->scalar_bytes, op->layout, epb);
+  return admissible(op->P, op->Q, op->qf, op->num_comp, op->scalar_bytes, op->layout, epb);
 }
+
+// Resident workgroups of a launch shape on the current device, cached by
+// (kernel, threads, LDS bytes, device).
 
 int64_t elem_block(const CeedMi355Operator op) {
   return (int64_t)op->q_comp * op->Q * op->Q * op->Q;
@@ -503,7 +520,7 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
     op->layout = mi355::kLayoutLines, op->epb = mi355::default_epb(P_1d, Q_1d, qf, op->scalar_bytes, op->layout);
   if (const char *env = getenv("CEED_MI355_EPB")) {
     const int v = atoi(env);
-    if (admissible(P_1d, Q_1d, qf, op->scalar_bytes, op->layout, v)) op->epb = v;
+    if (admissible(P_1d, Q_1d, qf, num_comp, op->scalar_bytes, op->layout, v)) op->epb = v;
   }
 
   const int            es = P_1d * P_1d * P_1d;
@@ -511,24 +528,28 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
   int                  num_colors = 0;
   int                  ierr       = CEED_MI355_SUCCESS;
   if (op->scatter == CEED_MI355_PLAN_BRICK) {
-    // bricks of face-sharing elements, one workgroup each; the shape by order
-    // (the brick's accumulator and node list share the CU's LDS with the slabs)
-    int shape[3] = {2, 2, 2};
-    if (P_1d >= 7) shape[1] = shape[2] = 1;
-    else if (P_1d == 6 || num_comp > 1) shape[2] = 1;
-    if (const char *env = getenv("CEED_MI355_BRICK")) {  // "bx,by,bz"
+    // bricks of face-sharing elements, one workgroup each: the largest of
+    // 2x2x2, 2x2x1, 2x1x1, 1x1x1 whose workgroup fits 512 threads and the
+    // CU's LDS; CEED_MI355_BRICK="bx,by,bz" replaces it
+    static const int candidates[4][3] = {{2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
+    int              shape[3]         = {1, 1, 1};
+    auto             fits             = [&](const int *sh) {
+      const int64_t ns = (int64_t)sh[0] * sh[1] * sh[2];
+      return mi355::brick_threads(Q_1d, ns) <= 512 && mi355::brick_lds_bytes(P_1d, Q_1d, qf, op->scalar_bytes, sh) <= mi355::kMaxLdsBytes;
+    };
+    for (const auto &c : candidates)
+      if (fits(c)) {
+        shape[0] = c[0], shape[1] = c[1], shape[2] = c[2];
+        break;
+      }
+    if (const char *env = getenv("CEED_MI355_BRICK")) {
       int a, b, c;
       if (sscanf(env, "%d,%d,%d", &a, &b, &c) == 3 && a >= 1 && b >= 1 && c >= 1 && a * b * c <= 8) shape[0] = a, shape[1] = b, shape[2] = c;
     }
-    if (const char *env = getenv("CEED_MI355_BRICK_PREFETCH")) op->brick_prefetch = atoi(env) != 0;
-    {  // the workgroup's LDS: slabs, accumulator, node list
-      const int64_t nb  = (int64_t)(shape[0] * (P_1d - 1) + 1) * (shape[1] * (P_1d - 1) + 1) * (shape[2] * (P_1d - 1) + 1);
-      const int64_t lds = op->scalar_bytes * ((qf == kQfMass ? 2 : 3) * (int64_t)Q_1d * Q_1d * Q_1d + num_comp * nb) + 4 * nb;
-      if (lds > mi355::kMaxLdsBytes) {
-        CeedMi355OperatorDestroy(&op);
-        MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED, "brick %dx%dx%d needs %lld bytes of LDS (at most %d)", shape[0], shape[1], shape[2],
-                    (long long)lds, mi355::kMaxLdsBytes);
-      }
+    if (!fits(shape)) {
+      CeedMi355OperatorDestroy(&op);
+      MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED, "brick %dx%dx%d needs more than 512 threads or %d bytes of LDS", shape[0], shape[1],
+                  shape[2], mi355::kMaxLdsBytes);
     }
     ierr = num_elem > 0 ? mi355::build_bricks(op, offsets_host, shape[0], shape[1], shape[2], op->perm) : CEED_MI355_SUCCESS;
     if (ierr == CEED_MI355_ERROR_UNSUPPORTED && !(flags & CEED_MI355_PLAN_BRICK)) {  // no bricks in this mesh: the default mode
@@ -776,7 +797,7 @@ extern "C" int CeedMi355OperatorSetLayout(CeedMi355Operator op, int32_t layout, 
   MI355_CHECK(layout == mi355::kLayoutLines || op->qf != kQfJit, CEED_MI355_ERROR_UNSUPPORTED,
               "plans of run-time compiled QFunctions have the line layout only");
   const int epb = elems_per_block > 0 ? elems_per_block : mi355::default_epb(op->P, op->Q, op->qf, op->scalar_bytes, layout);
-  MI355_CHECK(admissible(op->P, op->Q, op->qf, op->scalar_bytes, layout, epb), CEED_MI355_ERROR_BACKEND,
+  MI355_CHECK(admissible(op->P, op->Q, op->qf, op->num_comp, op->scalar_bytes, layout, epb), CEED_MI355_ERROR_BACKEND,
               "%d elements per workgroup do not fit layout %d (at most 256 threads and 160 KiB of LDS for P=%d Q=%d)", epb, layout, op->P, op->Q);
   op->layout = layout;
   op->epb    = epb;
@@ -814,7 +835,7 @@ extern "C" int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int
   int       err  = CEED_MI355_SUCCESS;
   for (int layout = mi355::kLayoutColumns; layout <= mi355::kLayoutLines && !err; layout++) {
     for (int epb = 1; epb * op->Q * op->Q <= mi355::max_threads(layout) && !err; epb++) {
-      if (!admissible(op->P, op->Q, op->qf, op->scalar_bytes, layout, epb)) continue;
+      if (!admissible(op->P, op->Q, op->qf, op->num_comp, op->scalar_bytes, layout, epb)) continue;
       op->layout = layout;
       op->epb    = epb;
       err        = CeedMi355OperatorApply(op, u, v, s);  // warm-up

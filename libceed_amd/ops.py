@@ -224,7 +224,7 @@ class HexOperator:
         call("CeedMi355OperatorSetLaunch", self._h, int(elems_per_block))
 
     def set_layout(self, layout: int, elems_per_block: int = 0) -> None:
-        """Thread layout of the fused kernel (LAYOUT_COLUMNS or LAYOUT_LINES); elems_per_block 0: as many as fit."""
+        """Thread layout of the fused kernel (LAYOUT_COLUMNS or LAYOUT_LINES); elems_per_block 0: the default."""
         call("CeedMi355OperatorSetLayout", self._h, int(layout), int(elems_per_block))
 
     def launch(self) -> tuple[int, int]:

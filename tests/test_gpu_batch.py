@@ -112,11 +112,7 @@ def test_batch_every_group_vs_oracle(cuda):
     """A batch holding every (mass or diffusion, p = 2..7) group of the batched workload, two meshes of
     different size per group (one of them a single element): every system's batched solution against the
     oracle's dense solve, at 1e-8 (diffusion compared modulo constants)."""
-    systems = []
-    for p in range(2, 8):
-        for kind in (ops.QF_MASS, ops.QF_DIFFUSION):
-            for n in (1, 2 if p >= 5 else 3):
-                systems.append(System(kind, hex_mesh(n, p, seed=97 * p + 13 * n + kind)))
+    systems = every_group_systems()
     bcg = BatchCG(systems)
     assert {(s.kind, s.mesh.P) for s in systems} == {(k, p + 1) for k in (ops.QF_MASS, ops.QF_DIFFUSION) for p in range(2, 8)}
     b = batch_rhs(bcg, seed=4)
@@ -132,3 +128,65 @@ def test_batch_every_group_vs_oracle(cuda):
         if s.kind == ops.QF_DIFFUSION:
             d = d - d.mean()
         assert np.linalg.norm(d) / np.linalg.norm(ref) < 1e-8, (i, s.kind, s.mesh.n, s.mesh.p)
+
+
+def every_group_systems():
+    systems = []
+    for p in range(2, 8):
+        for kind in (ops.QF_MASS, ops.QF_DIFFUSION):
+            for n in (1, 2 if p >= 5 else 3):
+                systems.append(System(kind, hex_mesh(n, p, seed=97 * p + 13 * n + kind)))
+    return systems
+
+
+@pytest.mark.skip(reason="persistent small-system CG switched off: its first GPU run ended in a GPU memory fault whose cause has not been found")
+def test_small_batch_every_group_vs_oracle(cuda):
+    """The persistent small-system launch (each workgroup solves whole systems off a work list) on every
+    (mass or diffusion, p = 2..7) group: each solution against the oracle's dense solve at 1e-8 (diffusion modulo
+    constants), true residuals at the tolerance, and a second solve gives identical bits."""
+    systems = every_group_systems()
+    bcg = BatchCG(systems, small_dofs=10 ** 9)
+    assert all(bcg.is_small(i) for i in range(len(systems)))
+    b = batch_rhs(bcg, seed=5)
+    xs = []
+    for _ in range(2):
+        x = torch.full_like(b, 3.0)  # the solver starts from x = 0 whatever x holds
+        st = bcg.solve(b, x, max_iter=5000, rtol=1e-12)
+        torch.cuda.synchronize()
+        xs.append(x.cpu().numpy())
+        assert np.all(st["rel_residual"] <= 1e-11), st["rel_residual"]
+        assert np.all(st["iterations"] > 0) and np.all(np.isfinite(st["done_ms"]))
+    assert np.array_equal(xs[0], xs[1])
+    bh = b.cpu().numpy()
+    for i, s in enumerate(systems):
+        r = bcg.range(i)
+        print("small batch vs oracle", i, s.kind, s.mesh.n, s.mesh.p, flush=True)
+        ref = oracle_solution(s.kind, s.mesh, bh[r])
+        d = xs[0][r] - ref
+        if s.kind == ops.QF_DIFFUSION:
+            d = d - d.mean()
+        assert np.linalg.norm(d) / np.linalg.norm(ref) < 1e-8, (i, s.kind, s.mesh.n, s.mesh.p)
+
+
+@pytest.mark.skip(reason="persistent small-system CG switched off: its first GPU run ended in a GPU memory fault whose cause has not been found")
+def test_hybrid_batch_matches_group_batched(cuda):
+    """Small systems on the persistent launch while the others run the group-batched CG on another stream:
+    every system converges, and each solution equals the group-batched solution of the same system."""
+    systems = random_systems(20, seed=9, max_n=5, p_range=(2, 5))
+    sizes = sorted(s.mesh.num_nodes for s in systems)
+    hyb = BatchCG(systems, small_dofs=sizes[len(sizes) // 2])
+    ref = BatchCG(systems)
+    assert 0 < hyb.num_large < len(systems)
+    b_ref = batch_rhs(ref, seed=6)
+    b = torch.empty_like(b_ref)
+    for i in range(len(systems)):  # the same right-hand sides in the hybrid's vector order
+        b[hyb.range(i)] = b_ref[ref.range(i)]
+    x, x_ref = torch.zeros_like(b), torch.zeros_like(b)
+    st = hyb.solve(b, x, max_iter=5000, rtol=1e-11, check_every=10)
+    ref.solve(b_ref, x_ref, max_iter=5000, rtol=1e-11, check_every=10)
+    assert np.all(st["rel_residual"] <= 1e-11), st["rel_residual"]
+    for i, s in enumerate(systems):
+        d = x[hyb.range(i)] - x_ref[ref.range(i)]
+        if s.kind == ops.QF_DIFFUSION:
+            d = d - d.mean()
+        assert (torch.linalg.norm(d) / torch.linalg.norm(x_ref[ref.range(i)])).item() < 1e-8, (i, hyb.is_small(i))

@@ -187,8 +187,9 @@ def test_operator_vs_oracle(cuda, kind, nc, n, p, Q):
                                                  (0, 3, 3, 2, 3, False), (1, 1, 5, 1, 2, False), (1, 1, 3, 5, 6, True), (0, 1, 3, 4, 5, True),
                                                  (2, 1, 3, 4, 5, False), (2, 1, 2, 6, 8, False)])
 def test_thread_layouts_identical(cuda, kind, nc, n, p, Q, fp32):
-    """Both thread layouts of the fused kernel (z-columns, 1D lines), every admissible elements-per-workgroup count:
-    bitwise identical results (so the autotuner's choice never changes them) and oracle parity."""
+    """Both thread layouts of the fused kernel (z-columns, 1D lines), every admissible elements-per-workgroup count
+    and every deterministic E^T mode: bitwise identical results across the launches (so the autotuner's choice
+    never changes them) and oracle parity."""
     from libceed_amd.mesh import quadrature_points, smooth_random_field
 
     m = hex_mesh(n, p, seed=3 * n + p)
@@ -201,7 +202,7 @@ def test_thread_layouts_identical(cuda, kind, nc, n, p, Q, fp32):
     else:
         v_ref = oracle.hex_apply(kind, nc, m.num_elem, p + 1, Q, m.offsets, m.num_nodes, qd_ref, u)
     dtype = torch.float32 if fp32 else torch.float64
-    for det in ("tmap", "color"):
+    for det in ("tmap", "color", "split") if nc == 1 else ("tmap", "color"):
         op = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, dtype=dtype, scatter=det)
         if kind == ops.QF_DIFFUSION_KAPPA:
             op.set_kappa(dev(kappa))
