@@ -93,7 +93,7 @@ def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0,
     s = 8 if dtype == torch.float64 else 4
     B = alg_bytes(m.num_nodes, m.num_elem, P, Q, ncomp=num_comp, s=s)
     mode = {"tmap": "transpose map", "color": "element colours", "split": "multiplicity split", "atomic": "atomics",
-            "brick": "element bricks"}[op.scatter_mode()]
+            "brick": "element bricks", "segment": "element segments"}[op.scatter_mode()]
     F = alg_flops(m.num_elem, P, Q, ncomp=num_comp)
     kname = KERNEL_OF_LAYOUT[layout]
     res = {
@@ -104,7 +104,9 @@ def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0,
                                           "at most two contributions, the ordered transpose map for the rest)",
                     "atomics": f"one apply = zero fill + {kname} (E^T by atomic adds)",
                     "element bricks": "one apply = hex_apply_brick_kernel (a workgroup per brick of elements, shared nodes summed "
-                                      "in LDS) + brick_gather_kernel (brick-surface nodes, partial sums in brick order)"}.get(
+                                      "in LDS) + brick_gather_kernel (brick-surface nodes, partial sums in brick order)",
+                    "element segments": f"one apply = {len(op.colors())} {kname} launches (one per colour of x-segments: a workgroup "
+                                        "per segment of face-sharing elements, the faces inside it summed in LDS)"}.get(
                         mode, f"one apply = {len(op.colors())} {kname} launches (one per element colour)"),
         "ms_per_apply": ms_apply, "wall_ms_per_apply": wall, "gdofs": num_comp * m.num_nodes / (ms_apply * 1e-3) / 1e9,
         # the roofline figures use Apply's kernel time (the operator's kernels of one apply, HIP events on
@@ -459,7 +461,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--scatter", choices=["auto", "tmap", "color", "split", "atomic", "brick"], default="auto",
+    ap.add_argument("--scatter", choices=["auto", "tmap", "color", "split", "atomic", "brick", "segment"], default="auto",
                     help="E^T mode of the headline plan (auto is the library default; the others are for profiling)")
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64",
                     help="arithmetic of the headline apply (the metric is FP64; f32 is for profiling)")

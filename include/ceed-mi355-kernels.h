@@ -50,7 +50,14 @@ enum { CEED_MI355_QF_MASS = 0, CEED_MI355_QF_DIFFUSION = 1, CEED_MI355_QF_DIFFUS
 enum { CEED_MI355_FIELD_INTERP = 0, CEED_MI355_FIELD_GRAD = 1, CEED_MI355_FIELD_PASSIVE = 2 };
 
 /* E^T mode flags of CeedMi355OperatorCreateHex (0: transpose map for P >= 6, colours below). */
-enum { CEED_MI355_PLAN_COLOR = 2, CEED_MI355_PLAN_ATOMIC = 4, CEED_MI355_PLAN_TMAP = 8, CEED_MI355_PLAN_SPLIT = 16, CEED_MI355_PLAN_BRICK = 32 };
+enum {
+  CEED_MI355_PLAN_COLOR   = 2,
+  CEED_MI355_PLAN_ATOMIC  = 4,
+  CEED_MI355_PLAN_TMAP    = 8,
+  CEED_MI355_PLAN_SPLIT   = 16,
+  CEED_MI355_PLAN_BRICK   = 32,
+  CEED_MI355_PLAN_SEGMENT = 64
+};
 
 /* Geometric-factor builders (gallery Mass3DBuild / Poisson3DBuild). */
 enum { CEED_MI355_BUILD_MASS = 0, CEED_MI355_BUILD_DIFFUSION = 1 };
@@ -114,7 +121,7 @@ typedef struct CeedMi355Operator_private *CeedMi355Operator;
  * one fused kernel launch.  E^T (flags; 0 picks the mode measured fastest on
  * MI355X for the order, precision and size: COLOR for P_1d <= 5 (FP64) or
  * <= 6 (FP32) from 10M E-vector entries, SPLIT for P_1d = 7, 8 in FP64 from
- * 10M entries, TMAP otherwise; CEED_MI355_SCATTER=color|tmap|split|atomic
+ * 10M entries, TMAP otherwise; CEED_MI355_SCATTER=color|tmap|split|atomic|brick|segment
  * replaces the choice):
  *   CEED_MI355_PLAN_TMAP   the fused kernel stores every contribution into
  *                          an E-vector sorted by L entry (slots from the
@@ -133,7 +140,19 @@ typedef struct CeedMi355Operator_private *CeedMi355Operator;
  *                          every entry summed in the reference's order, Apply
  *                          bitwise equal to TMAP; ApplyAdd forms A u in a
  *                          scratch vector, then adds it;
- *   CEED_MI355_PLAN_ATOMIC hardware atomic adds, sums in arrival order.
+ *   CEED_MI355_PLAN_ATOMIC hardware atomic adds, sums in arrival order;
+ *   CEED_MI355_PLAN_BRICK  a workgroup per brick of up to 2x2x2 face-sharing
+ *                          elements, the brick's shared nodes summed in LDS,
+ *                          brick-surface nodes finished by an ordered gather
+ *                          (mass and diffusion only);
+ *   CEED_MI355_PLAN_SEGMENT a workgroup per segment of up to 256 / Q_1d^2
+ *                          elements chained along x through shared faces;
+ *                          the face between consecutive elements is summed in
+ *                          LDS (left element's value first), segments are
+ *                          coloured so that no two of a colour share an L
+ *                          entry, one launch per colour adds in place (no
+ *                          atomics, bitwise reproducible; mass and diffusion
+ *                          Apply / ApplyAdd only, line thread layout).
  * Supports Q_1d in {P_1d, P_1d + 1}, P_1d in 2..9.  scalar_type selects FP64
  * or FP32 storage and arithmetic for u, v and q_data.  Replaces the
  * per-operator set-up of ceed-ref-operator.c:145-210. */

@@ -454,6 +454,8 @@ int launch_adjoint(const CeedMi355Operator_private *op, const double *u, const d
   const bool   lines = op->layout == kLayoutLines;
   const size_t lds   = sizeof(double) * (size_t)(lines ? epb * 5 * Q * Q * Q : Q * P + Q * Q + epb * 3 * Q * Q * Q);
   const bool   tmap  = op->scatter == 0;
+  MI355_CHECK(op->scatter != CEED_MI355_PLAN_SEGMENT, CEED_MI355_ERROR_UNSUPPORTED,
+              "the adjoint does not take segment plans (elements of a colour share segment faces)");
   MI355_CHECK(lds <= (size_t)kMaxLdsBytes, CEED_MI355_ERROR_BACKEND, "adjoint needs %zu bytes of LDS at %d elements per workgroup", lds, epb);
   for (size_t c = 0; c + 1 < op->color_start.size(); c++) {
     const int64_t b = op->color_start[c], e = op->color_start[c + 1];

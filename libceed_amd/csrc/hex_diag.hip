@@ -93,6 +93,8 @@ int launch_diag(const CeedMi355Operator_private *op, void *v, int add, hipStream
   const bool   tmap = op->scatter == 0;
   const bool   split = op->scatter == CEED_MI355_PLAN_SPLIT;
   const bool   brick = op->scatter == CEED_MI355_PLAN_BRICK;
+  MI355_CHECK(op->scatter != CEED_MI355_PLAN_SEGMENT, CEED_MI355_ERROR_UNSUPPORTED,
+              "diagonal assembly does not take segment plans (elements of a colour share segment faces)");
   if (brick) {  // element diagonals into the element-major E-vector, then the brick E^T
     if (!add && !op->covers_all) {
       diag_zero<T><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, (T *)v);

@@ -403,13 +403,17 @@ __device__ __forceinline__ void line_element(T *S0, T *S1, T *S2, int t, bool ac
 // 162 VGPRs, a wave per SIMD fewer, 20% slower on MI355X at 8M DoFs; P=5,
 // Q=6: 133 against 127).  Both compute every value with the same operations
 // in the same order (tests/test_gpu_brick.py::test_brick_single_brick_bitwise).
-template <int P, int Q, int NCOMP, int QF, typename T>
+// kSeg: the workgroup runs one segment of elements chained along x
+// (seg_start), summing the face between consecutive elements in LDS before
+// E^T; a template parameter so that the other modes keep their registers.
+template <int P, int Q, int NCOMP, int QF, typename T, bool kSeg = false>
 __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin, int64_t elem_end, int scatter,
                                                                const int32_t *__restrict__ offsets, int32_t comp_stride,
                                                                const T *__restrict__ q_data, const T *__restrict__ mats,
                                                                const T *__restrict__ u, T *__restrict__ v, T *__restrict__ evec,
                                                                const int32_t *__restrict__ tpos,
-                                                               const int32_t *__restrict__ elem_sys, const int32_t *__restrict__ sys_active) {
+                                                               const int32_t *__restrict__ elem_sys, const int32_t *__restrict__ sys_active,
+                                                               const int32_t *__restrict__ seg_start) {
   constexpr int NT = Q * Q, SLAB = Q * Q * Q, P3 = P * P * P, Q3 = Q * Q * Q, QC = QfTraits<QF>::kComps;
   constexpr int NS = LineSlabs<QF>::kSlabs;
   // from Q = 7 one phase's matrix entries no longer fit the SGPRs: load them
@@ -425,8 +429,13 @@ __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin
   T        *S2 = S1 + SLAB;  // diffusion only
   auto      at = [](int z, int y, int x) { return (z * Q + y) * Q + x; };
 
-  const int64_t e    = elem_begin + xcd_chunk(blockIdx.x, gridDim.x) * epb + le;
-  const bool    live = le < epb && e < elem_end && (!elem_sys || sys_active[elem_sys[e]]);
+  // the workgroup's elements: epb consecutive packed elements, or (kSeg)
+  // segment elem_begin + chunk, whose consecutive elements share x-faces
+  const int64_t chunk = xcd_chunk(blockIdx.x, gridDim.x);
+  const int32_t sbeg  = kSeg ? seg_start[elem_begin + chunk] : 0;
+  const int32_t slen  = kSeg ? seg_start[elem_begin + chunk + 1] - sbeg : 0;
+  const int64_t e     = kSeg ? (int64_t)sbeg + le : elem_begin + chunk * epb + le;
+  const bool    live  = le < epb && (kSeg ? le < slen : e < elem_end) && (!elem_sys || sys_active[elem_sys[e]]);
   if (elem_sys && !__syncthreads_or(live)) return;  // every element of the workgroup is frozen (uniform exit)
 
   const int  cx = t % Q, cy = t / Q;  // column
@@ -713,28 +722,42 @@ __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin
       T s[P];
 #pragma unroll
       for (int z = 0; z < P; z++) s[z] = S0[at(z, cy, cx)];
-      if (scatter == kScatterEvec) {  // tpos gives each contribution's slot in the E-vector sorted by L entry
+      if (!kSeg && scatter == kScatterEvec) {  // tpos gives each contribution's slot in the E-vector sorted by L entry
         int32_t pos[P];
 #pragma unroll
         for (int z = 0; z < P; z++) pos[z] = tpos[(e * NCOMP + comp) * (int64_t)P3 + (z * P + cy) * P + cx];
 #pragma unroll
         for (int z = 0; z < P; z++) evec[pos[z]] = s[z];
-      } else if (scatter == kScatterAtomic) {
+      } else if (!kSeg && scatter == kScatterAtomic) {
 #pragma unroll
         for (int z = 0; z < P; z++) unsafeAtomicAdd(&v[(off[z] >> 1) + coff], s[z]);
-      } else if (scatter == kScatterSplit) {  // shared entries to their E-vector slot, the rest onto the zeroed target
+      } else if (!kSeg && scatter == kScatterSplit) {  // shared entries to their E-vector slot, the rest onto the zeroed target
 #pragma unroll
         for (int z = 0; z < P; z++) {
           if (off[z] & 1) evec[tpos[(e * NCOMP + comp) * (int64_t)P3 + (z * P + cy) * P + cx]] = s[z];
           else unsafeAtomicAdd(&v[(off[z] >> 1) + coff], s[z]);
         }
       } else {
-        const bool overwrite = scatter == kScatterOverwrite;
-        T          old[P];
+        // segments: the face x = P-1 also holds the next element's x = 0 face
+        // (its S0 is final after the barrier above), added after the own value;
+        // the next element's threads of that face write nothing
+        bool write = true;
+        if constexpr (kSeg) {
+          if (cx == P - 1 && le + 1 < slen) {
+            const T *Sn = S0 + NS * SLAB;
 #pragma unroll
-        for (int z = 0; z < P; z++) old[z] = (overwrite && (off[z] & 1)) ? T(0) : v[(off[z] >> 1) + coff];
+            for (int z = 0; z < P; z++) s[z] += Sn[at(z, cy, 0)];
+          }
+          write = cx > 0 || le == 0;
+        }
+        if (write) {
+          const bool overwrite = scatter == kScatterOverwrite;
+          T          old[P];
 #pragma unroll
-        for (int z = 0; z < P; z++) v[(off[z] >> 1) + coff] = (overwrite && (off[z] & 1)) ? s[z] : old[z] + s[z];
+          for (int z = 0; z < P; z++) old[z] = (overwrite && (off[z] & 1)) ? T(0) : v[(off[z] >> 1) + coff];
+#pragma unroll
+          for (int z = 0; z < P; z++) v[(off[z] >> 1) + coff] = (overwrite && (off[z] & 1)) ? s[z] : old[z] + s[z];
+        }
       }
     }
   }

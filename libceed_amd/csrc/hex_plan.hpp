@@ -70,6 +70,19 @@ struct BrickLayout {
   std::vector<int32_t> perm, start, mask, bnode, blist, brow, bslot;
 };
 int brick_layout(const int32_t *offsets, int64_t num_elem, int P, int num_comp, int bx, int by, int bz, BrickLayout &out);
+// Segments of face-sharing elements along x for the segment E^T
+// (operator_brick.hip): chains of +x face neighbours cut into balanced pieces
+// of at most max_len elements (a piece also ends before an element that would
+// share a node with it other than through its -x face), coloured so that no
+// two segments of a colour share a node.  Returns CEED_MI355_ERROR_UNSUPPORTED
+// where an element repeats a node.
+struct SegmentLayout {
+  std::vector<int32_t> perm;         // packed position -> caller's element: colour by colour, segment by segment, along x
+  std::vector<int32_t> start;        // packed start of each segment in launch order, then the end
+  std::vector<int64_t> color;        // colour c owns segments [color[c], color[c+1])
+  int                  max_len = 0;  // the longest segment
+};
+int segment_layout(const int32_t *offsets, int64_t num_elem, int P, int max_len, SegmentLayout &out);
 // A plan of CeedMi355OperatorCreateHex with the given kernel launcher and
 // geometric-factor components per point (operator_hex.hip).
 int create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_elem, int32_t P_1d, int32_t Q_1d, const double *interp_1d_host,
@@ -87,7 +100,7 @@ struct CeedMi355Operator_private {
   int     epb;
   int     layout     = 0;      // mi355::kLayoutColumns or kLayoutLines
   bool    covers_all = false;  // every L entry is reached by some element: Apply needs no zero fill
-  int     scatter    = 0;      // CEED_MI355_PLAN_* E^T mode: 0 transpose map, COLOR, SPLIT, ATOMIC
+  int     scatter    = 0;      // CEED_MI355_PLAN_* E^T mode: 0 transpose map, COLOR, SPLIT, ATOMIC, BRICK, SEGMENT
   // transpose offset map (scatter == 0): contribution (elem, comp, node) is
   // stored at d_evec[d_tpos[(elem * num_comp + comp) * P^3 + node]], so that
   // L entry t owns the contiguous run d_evec[d_trow[t] .. d_trow[t+1]), in
@@ -128,6 +141,13 @@ struct CeedMi355Operator_private {
   int32_t *d_brick_start = nullptr, *d_brick_mask = nullptr, *d_bnode = nullptr;
   int32_t *d_blist = nullptr, *d_brow = nullptr, *d_bslot = nullptr;
   void    *d_partials = nullptr;
+  // segment mode (scatter == CEED_MI355_PLAN_SEGMENT): segment g owns the
+  // packed elements [d_seg_start[g], d_seg_start[g+1]), each sharing its x = 0
+  // face with the one before; colour c owns segments [seg_color[c],
+  // seg_color[c+1]); one workgroup per segment, seg_max element slots
+  int32_t             *d_seg_start = nullptr;
+  std::vector<int64_t> seg_color;
+  int                  seg_max = 0;
   // colour c owns packed elements [color_start[c], color_start[c+1])
   std::vector<int64_t> color_start;
   std::vector<int32_t> perm;  // packed position -> caller's element index
@@ -156,6 +176,19 @@ int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v
     MI355_LAUNCH_CHECK();
     return brick_gather(op, v, overwrite, s);
   }
+  if constexpr (QF == kQfMass || QF == kQfDiffusion) {  // segment plans: mass and diffusion, line layout only
+    if (op->scatter == CEED_MI355_PLAN_SEGMENT) {
+      for (size_t c = 0; c + 1 < op->seg_color.size(); c++) {
+        const int64_t b = op->seg_color[c], e = op->seg_color[c + 1];
+        if (e <= b) continue;
+        hex_apply_lines_kernel<P, Q, NC, QF, T, true><<<(unsigned)(e - b), epb * nt, lds, s>>>(
+            b, e, overwrite ? kScatterOverwrite : kScatterAdd, op->d_offsets, op->comp_stride, (const T *)op->d_qdata, (const T *)op->d_mats,
+            (const T *)u, (T *)v, nullptr, nullptr, op->d_elem_sys, op->d_sys_active, op->d_seg_start);
+      }
+      MI355_LAUNCH_CHECK();
+      return CEED_MI355_SUCCESS;
+    }
+  }
   const bool   tmap  = op->scatter == 0;
   const bool   split = op->scatter == CEED_MI355_PLAN_SPLIT;
   // the split mode's atomic adds land on a zeroed target: v in Apply (zeroed
@@ -175,7 +208,7 @@ int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v
     if (lines)
       hex_apply_lines_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(
           b, e, scatter, op->d_offsets, op->comp_stride, (const T *)op->d_qdata, (const T *)op->d_mats, (const T *)u, target, (T *)op->d_evec,
-          op->d_tpos, op->d_elem_sys, op->d_sys_active);
+          op->d_tpos, op->d_elem_sys, op->d_sys_active, nullptr);
     else
       hex_apply_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, scatter, op->d_offsets, op->comp_stride, (const T *)op->d_qdata,
                                                                             (const T *)op->d_mats, (const T *)u, target, (T *)op->d_evec,

@@ -1,6 +1,7 @@
-// Brick E^T for the fused hex plans: grouping the elements into bricks of
-// face-sharing elements (host), the gather that finishes the brick-surface
-// nodes, and the brick E^T of an element-major E-vector (diagonal assembly).
+// Brick and segment E^T for the fused hex plans: grouping the elements into
+// bricks of face-sharing elements (host), the gather that finishes the
+// brick-surface nodes, the brick E^T of an element-major E-vector (diagonal
+// assembly), and the x-segments of the segment E^T (host).
 //
 // Bricks are found from the element offsets alone (no coordinates): the
 // element whose local node (0, 0, 0) is the node at local (P-1, 0, 0) of e,
@@ -79,9 +80,164 @@ __global__ void __launch_bounds__(256) brick_evec_kernel(int P, int bx, int by, 
   }
 }
 
+// Elements by their local node (0, 0, 0): origin[node] is the element starting
+// there, -2 where two elements start at one node, -1 where none does.
+std::vector<int32_t> node_origins(const int32_t *offsets, int64_t ne, int P, int64_t n_nodes) {
+  const int            es = P * P * P;
+  std::vector<int32_t> origin((size_t)n_nodes, -1);
+  for (int64_t e = 0; e < ne; e++) {
+    int32_t &o = origin[offsets[e * es]];
+    o          = o == -1 ? (int32_t)e : -2;
+  }
+  return origin;
+}
+
+// Face neighbours along +d (d = 0, 1, 2 for x, y, z), checked node by node:
+// nxt[e] is the element whose whole local face at 0 along d equals e's face at
+// P - 1 (-1 if none); prv holds the inverse links (-2 where two elements
+// claim one neighbour).
+void face_links(const int32_t *offsets, int64_t ne, int P, const std::vector<int32_t> &origin, int d, std::vector<int32_t> &nxt,
+                std::vector<int32_t> &prv) {
+  const int p = P - 1, es = P * P * P;
+  auto      off = [&](int64_t e, int x, int y, int z) { return offsets[e * es + (z * P + y) * P + x]; };
+  nxt.assign((size_t)ne, -1);
+  prv.assign((size_t)ne, -1);
+  for (int64_t e = 0; e < ne; e++) {
+    const int32_t c = origin[d == 0 ? off(e, p, 0, 0) : d == 1 ? off(e, 0, p, 0) : off(e, 0, 0, p)];
+    if (c < 0 || c == e) continue;
+    bool match = true;
+    for (int a = 0; a < P && match; a++)
+      for (int b2 = 0; b2 < P && match; b2++) {
+        if (d == 0) match = off(c, 0, a, b2) == off(e, p, a, b2);
+        else if (d == 1) match = off(c, a, 0, b2) == off(e, a, p, b2);
+        else match = off(c, a, b2, 0) == off(e, a, b2, p);
+      }
+    if (!match) continue;
+    nxt[e] = c;
+    prv[c] = prv[c] == -1 ? (int32_t)e : -2;
+  }
+}
+
+// Position of every element along the chains of nxt links: steps from the
+// start of its chain (an element with no unique predecessor), or from an
+// arbitrary element of a chain that closes on itself (periodic meshes).
+std::vector<int32_t> chain_positions(const std::vector<int32_t> &nxt, const std::vector<int32_t> &prv) {
+  const int64_t        ne = (int64_t)nxt.size();
+  std::vector<int32_t> pos((size_t)ne, -1);
+  auto                 walk = [&](int64_t e) {
+    int32_t k = 0;
+    for (int64_t x = e; x >= 0 && pos[x] < 0; x = nxt[x]) pos[x] = k++;
+  };
+  for (int64_t e = 0; e < ne; e++)
+    if (prv[e] < 0) walk(e);
+  for (int64_t e = 0; e < ne; e++)
+    if (pos[e] < 0) walk(e);
+  return pos;
+}
+
 }  // namespace
 
 namespace mi355 {
+
+int segment_layout(const int32_t *offsets, int64_t num_elem, int P, int max_len, SegmentLayout &out) {
+  const int     es = P * P * P;
+  const int64_t ne = num_elem;
+  MI355_CHECK(max_len >= 1 && P >= 2, CEED_MI355_ERROR_UNSUPPORTED, "segments of %d elements at P = %d", max_len, P);
+  int64_t n_nodes = 0;
+  for (int64_t i = 0; i < ne * es; i++) n_nodes = std::max<int64_t>(n_nodes, (int64_t)offsets[i] + 1);
+  const std::vector<int32_t> origin = node_origins(offsets, ne, P, n_nodes);
+  std::vector<int32_t>       nxt, prv, ynxt, yprv, znxt, zprv;
+  face_links(offsets, ne, P, origin, 0, nxt, prv);
+  face_links(offsets, ne, P, origin, 1, ynxt, yprv);
+  face_links(offsets, ne, P, origin, 2, znxt, zprv);
+  const std::vector<int32_t> ypos = chain_positions(ynxt, yprv), zpos = chain_positions(znxt, zprv);
+  // chains along +x: from every element without a unique -x neighbour, then
+  // from what is left (chains that close on themselves)
+  std::vector<uint8_t>              in_chain((size_t)ne, 0);
+  std::vector<std::vector<int32_t>> chains;
+  auto                              walk = [&](int64_t h) {
+    std::vector<int32_t> c;
+    for (int64_t x = h, prev = -1; x >= 0 && !in_chain[x] && (prev < 0 || prv[x] == prev); prev = x, x = nxt[x]) {
+      c.push_back((int32_t)x);
+      in_chain[x] = 1;
+    }
+    chains.push_back(std::move(c));
+  };
+  for (int64_t e = 0; e < ne; e++)
+    if (prv[e] < 0 && !in_chain[e]) walk(e);
+  for (int64_t e = 0; e < ne; e++)
+    if (!in_chain[e]) walk(e);
+  // chains in (z, y) position order, so that the greedy colouring below meets
+  // the segments row by row whatever the element numbering (8 colours on a
+  // structured mesh)
+  std::stable_sort(chains.begin(), chains.end(), [&](const std::vector<int32_t> &a, const std::vector<int32_t> &b) {
+    return std::make_pair(zpos[a[0]], ypos[a[0]]) < std::make_pair(zpos[b[0]], ypos[b[0]]);
+  });
+  // balanced pieces of at most max_len; a piece also ends before an element
+  // that would share a node with it other than through its -x face
+  std::vector<std::vector<int32_t>> segs;
+  std::vector<int32_t>              stamp((size_t)n_nodes, -1);
+  for (const auto &c : chains) {
+    const int64_t L = (int64_t)c.size(), k = (L + max_len - 1) / max_len;
+    int64_t       at = 0;
+    for (int64_t piece = 0; piece < k; piece++) {
+      const int64_t len = L / k + (piece < L % k ? 1 : 0), end = at + len;
+      while (at < end) {
+        const int32_t        id = (int32_t)segs.size();
+        std::vector<int32_t> seg;
+        for (; at < end; at++) {
+          const int32_t e     = c[at];
+          const bool    first = seg.empty();
+          bool          ok    = true;
+          for (int i = 0; i < es && ok; i++) {
+            const bool face = !first && i % P == 0;  // the x = 0 face, shared with the element before
+            const int32_t g = offsets[(int64_t)e * es + i];
+            if (face ? stamp[g] != id : stamp[g] == id) ok = false;
+          }
+          if (!ok && !first) break;
+          MI355_CHECK(ok, CEED_MI355_ERROR_UNSUPPORTED, "element %d repeats a node: no segment E^T for this mesh", e);
+          for (int i = 0; i < es; i++) stamp[offsets[(int64_t)e * es + i]] = id;
+          seg.push_back(e);
+        }
+        segs.push_back(std::move(seg));
+      }
+    }
+  }
+  // colours: no two segments of a colour share a node (greedy, in segment order)
+  std::vector<uint64_t> used_by((size_t)n_nodes, 0);
+  std::vector<int32_t>  color(segs.size(), 0);
+  int                   num_colors = 0;
+  for (size_t g = 0; g < segs.size(); g++) {
+    uint64_t used = 0;
+    for (int32_t e : segs[g])
+      for (int i = 0; i < es; i++) used |= used_by[offsets[(int64_t)e * es + i]];
+    MI355_CHECK(~used != 0, CEED_MI355_ERROR_UNSUPPORTED, "segment colouring needs more than 64 colours");
+    int k = 0;
+    while (used & (1ull << k)) k++;
+    color[g]   = k;
+    num_colors = std::max(num_colors, k + 1);
+    for (int32_t e : segs[g])
+      for (int i = 0; i < es; i++) used_by[offsets[(int64_t)e * es + i]] |= 1ull << k;
+  }
+  // packed order: colour by colour, segments in order, elements along the chain
+  out.perm.clear();
+  out.perm.reserve((size_t)ne);
+  out.start.clear();
+  out.color.assign((size_t)num_colors + 1, 0);
+  out.max_len = 0;
+  for (int k = 0; k < num_colors; k++) {
+    for (size_t g = 0; g < segs.size(); g++) {
+      if (color[g] != k) continue;
+      out.start.push_back((int32_t)out.perm.size());
+      out.perm.insert(out.perm.end(), segs[g].begin(), segs[g].end());
+      out.max_len = std::max(out.max_len, (int)segs[g].size());
+    }
+    out.color[k + 1] = (int64_t)out.start.size();
+  }
+  out.start.push_back((int32_t)out.perm.size());
+  MI355_CHECK((int64_t)out.perm.size() == ne, CEED_MI355_ERROR_BACKEND, "segment packing lost elements");
+  return CEED_MI355_SUCCESS;
+}
 
 int brick_layout(const int32_t *offsets, int64_t num_elem, int P, int num_comp, int bx, int by, int bz, BrickLayout &out) {
   const int     p = P - 1, es = P * P * P, nc = num_comp;
@@ -91,44 +247,12 @@ int brick_layout(const int32_t *offsets, int64_t num_elem, int P, int num_comp, 
   auto off = [&](int64_t e, int x, int y, int z) { return offsets[e * es + (z * P + y) * P + x]; };
   int64_t n_nodes = 0;
   for (int64_t i = 0; i < ne * es; i++) n_nodes = std::max<int64_t>(n_nodes, (int64_t)offsets[i] + 1);
-  // elements by their local node (0, 0, 0); -2 where two elements start at one node
-  std::vector<int32_t> origin((size_t)n_nodes, -1);
-  for (int64_t e = 0; e < ne; e++) {
-    int32_t &o = origin[off(e, 0, 0, 0)];
-    o          = o == -1 ? (int32_t)e : -2;
-  }
   // face neighbours along +x, +y, +z, checked node by node, and the inverse links
-  std::vector<int32_t> nxt[3], prv[3], pos[3];
+  const std::vector<int32_t> origin = node_origins(offsets, ne, P, n_nodes);
+  std::vector<int32_t>       nxt[3], prv[3], pos[3];
   for (int d = 0; d < 3; d++) {
-    nxt[d].assign((size_t)ne, -1);
-    prv[d].assign((size_t)ne, -1);
-    pos[d].assign((size_t)ne, -1);
-  }
-  for (int64_t e = 0; e < ne; e++)
-    for (int d = 0; d < 3; d++) {
-      const int32_t c = origin[d == 0 ? off(e, p, 0, 0) : d == 1 ? off(e, 0, p, 0) : off(e, 0, 0, p)];
-      if (c < 0 || c == e) continue;
-      bool match = true;
-      for (int a = 0; a < P && match; a++)
-        for (int b2 = 0; b2 < P && match; b2++) {
-          if (d == 0) match = off(c, 0, a, b2) == off(e, p, a, b2);
-          else if (d == 1) match = off(c, a, 0, b2) == off(e, a, p, b2);
-          else match = off(c, a, b2, 0) == off(e, a, b2, p);
-        }
-      if (!match) continue;
-      nxt[d][e] = c;
-      prv[d][c] = prv[d][c] == -1 ? (int32_t)e : -2;
-    }
-  // position of each element along each direction: steps from the start of its chain
-  for (int d = 0; d < 3; d++) {
-    auto walk = [&](int64_t e) {
-      int32_t k = 0;
-      for (int64_t x = e; x >= 0 && pos[d][x] < 0; x = nxt[d][x]) pos[d][x] = k++;
-    };
-    for (int64_t e = 0; e < ne; e++)
-      if (prv[d][e] < 0) walk(e);
-    for (int64_t e = 0; e < ne; e++)  // chains that close on themselves (periodic meshes)
-      if (pos[d][e] < 0) walk(e);
+    face_links(offsets, ne, P, origin, d, nxt[d], prv[d]);
+    pos[d] = chain_positions(nxt[d], prv[d]);
   }
   // bricks
   const int nbx = bx * p + 1, nby = by * p + 1, nbz = bz * p + 1, nb_nodes = nbx * nby * nbz, ns = bx * by * bz;

@@ -256,8 +256,11 @@ bool admissible(int P, int Q, int qf, int num_comp, int scalar_bytes, int layout
          epb * Q * Q <= mi355::max_threads(layout) && mi355::lds_bytes(P, Q, qf, epb, scalar_bytes, layout) <= mi355::kMaxLdsBytes;
 }
 
-bool admissible_epb(const CeedMi355Operator op, int epb) {
-  return admissible(op->P, op->Q, op->qf, op->num_comp, op->scalar_bytes, op->layout, epb);
+// A launch the plan can take: segment plans have one (the line layout with an
+// element slot for each element of the longest segment)
+bool plan_admissible(const CeedMi355Operator op, int layout, int epb) {
+  if (op->scatter == CEED_MI355_PLAN_SEGMENT) return layout == mi355::kLayoutLines && epb == op->seg_max;
+  return admissible(op->P, op->Q, op->qf, op->num_comp, op->scalar_bytes, layout, epb);
 }
 
 // Resident workgroups of a launch shape on the current device, cached by
@@ -470,34 +473,37 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
   // at p = 6, 1.03x at p = 7, 8M DoFs.
   const int64_t evec_entries = (int64_t)num_elem * P_1d * P_1d * P_1d * num_comp;
   const bool    big          = evec_entries >= 10000000;
-  if (flags & CEED_MI355_PLAN_BRICK) op->scatter = CEED_MI355_PLAN_BRICK;
+  if (flags & CEED_MI355_PLAN_SEGMENT) op->scatter = CEED_MI355_PLAN_SEGMENT;
+  else if (flags & CEED_MI355_PLAN_BRICK) op->scatter = CEED_MI355_PLAN_BRICK;
   else if (flags & CEED_MI355_PLAN_SPLIT) op->scatter = CEED_MI355_PLAN_SPLIT;
   else if (flags & CEED_MI355_PLAN_ATOMIC) op->scatter = CEED_MI355_PLAN_ATOMIC;
   else if (flags & CEED_MI355_PLAN_COLOR) op->scatter = CEED_MI355_PLAN_COLOR;
   else if (flags & CEED_MI355_PLAN_TMAP) op->scatter = 0;
   else if (big && scalar_type == CEED_MI355_FP64 && (P_1d == 7 || P_1d == 8)) op->scatter = CEED_MI355_PLAN_SPLIT;
   else op->scatter = P_1d >= (scalar_type == CEED_MI355_FP32 ? 7 : 6) || !big ? 0 : CEED_MI355_PLAN_COLOR;
-  const int explicit_modes = CEED_MI355_PLAN_ATOMIC | CEED_MI355_PLAN_COLOR | CEED_MI355_PLAN_TMAP | CEED_MI355_PLAN_SPLIT | CEED_MI355_PLAN_BRICK;
+  const int explicit_modes =
+      CEED_MI355_PLAN_ATOMIC | CEED_MI355_PLAN_COLOR | CEED_MI355_PLAN_TMAP | CEED_MI355_PLAN_SPLIT | CEED_MI355_PLAN_BRICK | CEED_MI355_PLAN_SEGMENT;
   const int default_scatter = op->scatter;
   if (!(flags & explicit_modes))
-    if (const char *env = getenv("CEED_MI355_SCATTER")) {  // color | tmap | split | atomic | brick replaces the default
+    if (const char *env = getenv("CEED_MI355_SCATTER")) {  // color | tmap | split | atomic | brick | segment replaces the default
       if (!strcmp(env, "color")) op->scatter = CEED_MI355_PLAN_COLOR;
       else if (!strcmp(env, "tmap")) op->scatter = 0;
       else if (!strcmp(env, "split")) op->scatter = CEED_MI355_PLAN_SPLIT;
       else if (!strcmp(env, "atomic")) op->scatter = CEED_MI355_PLAN_ATOMIC;
       else if (!strcmp(env, "brick")) op->scatter = CEED_MI355_PLAN_BRICK;
+      else if (!strcmp(env, "segment")) op->scatter = CEED_MI355_PLAN_SEGMENT;
       else {
         delete op;
-        MI355_CHECK(false, CEED_MI355_ERROR_BACKEND, "CEED_MI355_SCATTER=%s: expected color, tmap, split, atomic or brick", env);
+        MI355_CHECK(false, CEED_MI355_ERROR_BACKEND, "CEED_MI355_SCATTER=%s: expected color, tmap, split, atomic, brick or segment", env);
       }
     }
-  // the brick E^T exists for the gallery mass and diffusion kernels; a plan of
-  // another kind asked for it by flag is an error, by the environment it keeps
-  // its default
-  if (op->scatter == CEED_MI355_PLAN_BRICK && !(qf == kQfMass || qf == kQfDiffusion)) {
-    if (flags & CEED_MI355_PLAN_BRICK) {
+  // the brick and segment E^T exist for the gallery mass and diffusion
+  // kernels; a plan of another kind asked for one by flag is an error, by the
+  // environment it keeps its default
+  if ((op->scatter == CEED_MI355_PLAN_BRICK || op->scatter == CEED_MI355_PLAN_SEGMENT) && !(qf == kQfMass || qf == kQfDiffusion)) {
+    if (flags & (CEED_MI355_PLAN_BRICK | CEED_MI355_PLAN_SEGMENT)) {
       delete op;
-      MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED, "the brick E^T supports mass and diffusion plans only");
+      MI355_CHECK(false, CEED_MI355_ERROR_UNSUPPORTED, "the brick and segment E^T support mass and diffusion plans only");
     }
     op->scatter = default_scatter;
   }
@@ -567,9 +573,33 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
       return ierr;
     }
   }
-  const bool colored = op->scatter == CEED_MI355_PLAN_COLOR;
-  const bool bricks  = op->scatter == CEED_MI355_PLAN_BRICK;
-  if (colored) ierr = color_elements(offsets_host, num_elem, es, num_comp, comp_stride, l_size, color, num_colors);
+  const bool colored  = op->scatter == CEED_MI355_PLAN_COLOR;
+  const bool bricks   = op->scatter == CEED_MI355_PLAN_BRICK;
+  const bool segments = op->scatter == CEED_MI355_PLAN_SEGMENT;
+  std::vector<uint8_t> seg_first;  // segments: packed element starts its segment
+  if (segments) {
+    // a workgroup per segment in the line layout, as many element slots as
+    // 256 threads hold (within the CU's LDS), no other launch
+    int max_len = 256 / (Q_1d * Q_1d);
+    while (max_len > 1 && mi355::lds_bytes(P_1d, Q_1d, qf, max_len, op->scalar_bytes, mi355::kLayoutLines) > mi355::kMaxLdsBytes) max_len--;
+    mi355::SegmentLayout L;
+    ierr = max_len >= 1 ? mi355::segment_layout(offsets_host, num_elem, P_1d, max_len, L) : CEED_MI355_ERROR_UNSUPPORTED;
+    if (ierr) {
+      delete op;
+      return ierr;
+    }
+    op->perm    = std::move(L.perm);
+    op->seg_color = std::move(L.color);
+    op->seg_max = std::max(1, L.max_len);
+    op->layout  = mi355::kLayoutLines;
+    op->epb     = op->seg_max;
+    op->color_start.assign(op->seg_color.size(), 0);
+    for (size_t c = 0; c < op->seg_color.size(); c++) op->color_start[c] = L.start[op->seg_color[c]];
+    seg_first.assign((size_t)num_elem, 0);
+    for (size_t g = 0; g + 1 < L.start.size(); g++) seg_first[L.start[g]] = 1;
+    MI355_HIP(hipMalloc(&op->d_seg_start, sizeof(int32_t) * L.start.size()));
+    MI355_HIP(hipMemcpy(op->d_seg_start, L.start.data(), sizeof(int32_t) * L.start.size(), hipMemcpyHostToDevice));
+  } else if (colored) ierr = color_elements(offsets_host, num_elem, es, num_comp, comp_stride, l_size, color, num_colors);
   else {  // one launch over all elements in element order (brick order for bricks)
     color.assign((size_t)num_elem, 0);
     num_colors = num_elem > 0 ? 1 : 0;
@@ -582,7 +612,7 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
     op->color_start.assign(2, 0);
     op->color_start[1] = num_elem;
     if (num_elem == 0) op->color_start.resize(1);
-  } else {
+  } else if (!segments) {  // (segments: packed order and colours from the segment layout)
     // packed order: by colour, ascending element index inside a colour
     op->color_start.assign(num_colors + 1, 0);
     for (int64_t e = 0; e < num_elem; e++) op->color_start[color[e] + 1]++;
@@ -591,9 +621,11 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
     std::vector<int64_t> cursor(op->color_start.begin(), op->color_start.end() - 1);
     for (int64_t e = 0; e < num_elem; e++) op->perm[cursor[color[e]]++] = (int32_t)e;
   }
-  // offsets in colour order, encoded 2 * node + bit: with colours the bit marks the first
-  // element (in launch order) reaching a node, which may overwrite v in CeedOperatorApply
-  // instead of accumulating; in the split mode it marks nodes of more than two contributions
+  // offsets in colour order, encoded 2 * node + bit: with colours (and segments) the bit marks
+  // the first element (in launch order) writing a node, which may overwrite v in CeedOperatorApply
+  // instead of accumulating; in the split mode it marks nodes of more than two contributions.
+  // In a segment, the x = 0 face of every element but the first is written by the element
+  // before it, so it neither writes nor counts as reaching the node.
   std::vector<int32_t> packed((size_t)num_elem * es);
   const bool           split = op->scatter == CEED_MI355_PLAN_SPLIT;
   {
@@ -609,9 +641,10 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
       for (int64_t i = 0; i < (int64_t)num_elem * es; i++) mult[offsets_host[i]]++;
     for (int64_t p = 0; p < num_elem; p++)
       for (int i = 0; i < es; i++) {
-        const int32_t node = offsets_host[(int64_t)op->perm[p] * es + i];
-        packed[p * es + i] = 2 * node + ((colored && !touched[node]) || (split && mult[node] > 2) ? 1 : 0);
-        touched[node]      = 1;
+        const int32_t node     = offsets_host[(int64_t)op->perm[p] * es + i];
+        const bool    absorbed = segments && !seg_first[p] && i % P_1d == 0;
+        packed[p * es + i]     = 2 * node + (((colored || segments) && !absorbed && !touched[node]) || (split && mult[node] > 2) ? 1 : 0);
+        if (!absorbed) touched[node] = 1;
       }
     std::vector<uint8_t> hit(l_size, 0);
     for (int64_t n = 0; n < n_nodes; n++)
@@ -625,7 +658,7 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
           h = 1;
         }
     // the transpose map writes every L entry; atomics need a zeroed v; colours and bricks overwrite what they reach
-    op->covers_all = op->scatter == 0 || ((colored || bricks) && std::all_of(hit.begin(), hit.end(), [](uint8_t h) { return h != 0; }));
+    op->covers_all = op->scatter == 0 || ((colored || bricks || segments) && std::all_of(hit.begin(), hit.end(), [](uint8_t h) { return h != 0; }));
   }
   if (split) {
     ierr = build_split_map(op, packed.data());
@@ -785,7 +818,7 @@ extern "C" int CeedMi355OperatorApply(CeedMi355Operator op, const void *u, void 
 
 extern "C" int CeedMi355OperatorSetLaunch(CeedMi355Operator op, int32_t elems_per_block) {
   MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
-  MI355_CHECK(admissible_epb(op, elems_per_block), CEED_MI355_ERROR_BACKEND,
+  MI355_CHECK(plan_admissible(op, op->layout, elems_per_block), CEED_MI355_ERROR_BACKEND,
               "%d elements per workgroup do not fit (at most 256 threads and 160 KiB of LDS for P=%d Q=%d)", elems_per_block, op->P, op->Q);
   op->epb = elems_per_block;
   return CEED_MI355_SUCCESS;
@@ -796,8 +829,10 @@ extern "C" int CeedMi355OperatorSetLayout(CeedMi355Operator op, int32_t layout, 
   MI355_CHECK(layout == mi355::kLayoutColumns || layout == mi355::kLayoutLines, CEED_MI355_ERROR_BACKEND, "unknown thread layout %d", layout);
   MI355_CHECK(layout == mi355::kLayoutLines || op->qf != kQfJit, CEED_MI355_ERROR_UNSUPPORTED,
               "plans of run-time compiled QFunctions have the line layout only");
-  const int epb = elems_per_block > 0 ? elems_per_block : mi355::default_epb(op->P, op->Q, op->qf, op->scalar_bytes, layout);
-  MI355_CHECK(admissible(op->P, op->Q, op->qf, op->num_comp, op->scalar_bytes, layout, epb), CEED_MI355_ERROR_BACKEND,
+  const int epb = elems_per_block > 0                       ? elems_per_block
+                  : op->scatter == CEED_MI355_PLAN_SEGMENT ? op->seg_max
+                                                           : mi355::default_epb(op->P, op->Q, op->qf, op->scalar_bytes, layout);
+  MI355_CHECK(plan_admissible(op, layout, epb), CEED_MI355_ERROR_BACKEND,
               "%d elements per workgroup do not fit layout %d (at most 256 threads and 160 KiB of LDS for P=%d Q=%d)", epb, layout, op->P, op->Q);
   op->layout = layout;
   op->epb    = epb;
@@ -835,7 +870,7 @@ extern "C" int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int
   int       err  = CEED_MI355_SUCCESS;
   for (int layout = mi355::kLayoutColumns; layout <= mi355::kLayoutLines && !err; layout++) {
     for (int epb = 1; epb * op->Q * op->Q <= mi355::max_threads(layout) && !err; epb++) {
-      if (!admissible(op->P, op->Q, op->qf, op->num_comp, op->scalar_bytes, layout, epb)) continue;
+      if (!plan_admissible(op, layout, epb)) continue;
       op->layout = layout;
       op->epb    = epb;
       err        = CeedMi355OperatorApply(op, u, v, s);  // warm-up
@@ -899,6 +934,7 @@ extern "C" int CeedMi355OperatorDestroy(CeedMi355Operator *op) {
   hipFree((*op)->d_brow);
   hipFree((*op)->d_bslot);
   hipFree((*op)->d_partials);
+  hipFree((*op)->d_seg_start);
   delete *op;
   *op = nullptr;
   return CEED_MI355_SUCCESS;

@@ -24,8 +24,8 @@ FP64, FP32 = 0, 1
 # E^T of the fused plan: "auto" (colours below P = 6 in FP64 / P = 7 in FP32 and the multiplicity split at P = 7, 8 in
 # FP64, both on meshes of at least 10M E-vector entries; the transpose map otherwise), transpose offset map, element
 # colours, multiplicity split (atomic adds for entries of at most two contributions, the ordered map for the rest),
-# hardware atomics
-SCATTER_FLAGS = {"auto": 0, "tmap": 8, "color": 2, "atomic": 4, "split": 16, "brick": 32}
+# hardware atomics, bricks of face-sharing elements, coloured x-segments of face-sharing elements
+SCATTER_FLAGS = {"auto": 0, "tmap": 8, "color": 2, "atomic": 4, "split": 16, "brick": 32, "segment": 64}
 GAUSS, GAUSS_LOBATTO = 0, 1
 
 
@@ -240,7 +240,7 @@ class HexOperator:
         return best.value, ms.value
 
     def scatter_mode(self) -> str:
-        """The plan's E^T mode: "tmap", "color", "split", "atomic" or "brick"."""
+        """The plan's E^T mode: "tmap", "color", "split", "atomic", "brick" or "segment"."""
         mode = C.c_int32()
         call("CeedMi355OperatorGetScatter", self._h, C.byref(mode))
         return {v: k for k, v in SCATTER_FLAGS.items() if k != "auto"}[mode.value]

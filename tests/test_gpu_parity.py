@@ -163,7 +163,7 @@ def test_operator_vs_oracle(cuda, kind, nc, n, p, Q):
     u = rng.uniform(-1, 1, nc * m.num_nodes)
     qd_ref = oracle.hex_build_qdata(kind, m, Q)
     v_ref = oracle.hex_apply(kind, nc, m.num_elem, p + 1, Q, m.offsets, m.num_nodes, qd_ref, u)
-    for det in ("tmap", "color", "atomic", "brick"):  # the E^T modes of the fused plan
+    for det in ("tmap", "color", "atomic", "brick", "segment"):  # the E^T modes of the fused plan
         # geometric factors built on the GPU
         op = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, scatter=det)
         assert rel_l2(host(op.get_qdata()), qd_ref) < RTOL64
@@ -268,6 +268,48 @@ def test_split_scatter(cuda, kind, nc, n, p, Q, fp32, layout):
     assert rel_l2(out["split"][2], w0 + v_ref) < tol
 
 
+@pytest.mark.parametrize("kind,nc,n,p,Q,fp32", [(1, 1, 9, 4, 6, False), (0, 1, 6, 3, 5, False), (1, 3, 5, 3, 5, False), (1, 1, 4, 6, 8, True),
+                                                 (1, 1, 3, 8, 10, False), (0, 3, 4, 2, 3, True), (1, 1, 8, 1, 2, False), (1, 1, 15, 4, 6, False)])
+def test_segment_scatter(cuda, kind, nc, n, p, Q, fp32):
+    """Segments of face-sharing elements along x, a workgroup each, the faces inside a segment summed in LDS and
+    the segments coloured: Apply and ApplyAdd against the oracle, bitwise repeatable; the plan's one launch (line
+    layout, a slot per element of the longest segment: every x-row of n elements cut into ceil(n / (256 / Q^2))
+    balanced pieces), 8 colours on a structured mesh, the autotuner keeping that launch, other launches and the
+    diagonal refused."""
+    m = hex_mesh(n, p, seed=5 * n + p)
+    rng = np.random.default_rng(7 * p + nc)
+    u = rng.uniform(-1, 1, nc * m.num_nodes)
+    qd_ref = oracle.hex_build_qdata(kind, m, Q)
+    v_ref = oracle.hex_apply(kind, nc, m.num_elem, p + 1, Q, m.offsets, m.num_nodes, qd_ref, u)
+    dtype = torch.float32 if fp32 else torch.float64
+    tol = RTOL32 if fp32 else RTOL64
+    op = ops.HexOperator.from_mesh(kind, m, Q=Q, num_comp=nc, dtype=dtype, scatter="segment")
+    assert op.scatter_mode() == "segment"
+    k = -(-n // (256 // (Q * Q)))
+    assert op.launch() == (ops.LAYOUT_LINES, -(-n // k))
+    assert len(op.colors()) == (8 if k > 1 else 4)
+    v = torch.full((nc * m.num_nodes,), 3.0, dtype=dtype, device="cuda")
+    op.apply(dev(u, dtype), v)
+    first = v.clone()
+    op.apply(dev(u, dtype), v)
+    assert torch.equal(first, v)
+    assert rel_l2(host(v), v_ref) < tol
+    w0 = rng.uniform(-1, 1, u.size)
+    w = dev(w0, dtype)
+    op.apply_add(dev(u, dtype), w)
+    assert rel_l2(host(w), w0 + v_ref) < tol
+    launch = op.launch()
+    op.autotune(2)
+    assert op.launch() == launch
+    with pytest.raises(ops.CeedMi355Error):
+        op.set_layout(ops.LAYOUT_COLUMNS, 1)
+    with pytest.raises(ops.CeedMi355Error):
+        op.set_launch(launch[1] + 1)
+    if nc == 1:
+        with pytest.raises(ops.CeedMi355Error):
+            op.assemble_diagonal(torch.zeros(m.num_nodes, dtype=dtype, device="cuda"))
+
+
 def test_mass_config_repeated_apply(cuda):
     """Mass operator, 16^3 elements, p = 3, Q = 5: v = M u applied 100 times, bitwise repeatable, oracle parity."""
     m = hex_mesh(16, 3, seed=1)
@@ -275,7 +317,7 @@ def test_mass_config_repeated_apply(cuda):
     qd = oracle.hex_build_qdata(0, m, 5)
     v_ref = oracle.hex_apply(0, 1, m.num_elem, 4, 5, m.offsets, m.num_nodes, qd, u)
     ud, v = dev(u), torch.zeros(m.num_nodes, dtype=torch.float64, device="cuda")
-    for scatter in ("tmap", "color", "brick"):  # the atomic-free E^T modes: bitwise repeatable
+    for scatter in ("tmap", "color", "brick", "segment"):  # the atomic-free E^T modes: bitwise repeatable
         op = ops.HexOperator.from_mesh(ops.QF_MASS, m, Q=5, scatter=scatter)
         op.apply(ud, v)
         first = v.clone()
@@ -359,7 +401,7 @@ def test_default_modes_full_size(cuda, p, mode):
     assert Ab.abs().max().item() < 1e-10
 
 
-@pytest.mark.parametrize("scatter", ["tmap", "color", "split", "atomic", "brick"])
+@pytest.mark.parametrize("scatter", ["tmap", "color", "split", "atomic", "brick", "segment"])
 @pytest.mark.parametrize("kind,nc,p", [(1, 1, 3), (0, 1, 4), (1, 3, 2)])
 def test_operator_unstructured_numbering(cuda, scatter, kind, nc, p):
     """Nodes renumbered by a random permutation into an L-vector with unused entries, elements in random order
@@ -396,7 +438,7 @@ def test_operator_unstructured_numbering(cuda, scatter, kind, nc, p):
     assert rel_l2(wh, w0 + v_ref) < RTOL64
 
 
-@pytest.mark.parametrize("scatter", ["tmap", "color", "split", "atomic", "brick"])
+@pytest.mark.parametrize("scatter", ["tmap", "color", "split", "atomic", "brick", "segment"])
 def test_operator_no_elements(cuda, scatter):
     """A plan with no elements: Apply gives zeros, ApplyAdd leaves v unchanged."""
     op = ops.HexOperator(ops.QF_DIFFUSION, np.zeros(0, dtype=np.int32), 0, 3, 4, 10, scatter=scatter)
@@ -425,11 +467,11 @@ def headline_reference(p):
     return _HEADLINE[p]
 
 
-@pytest.mark.parametrize("p,scatter", [(4, "auto"), (4, "brick"), (6, "auto"), (6, "brick")])
+@pytest.mark.parametrize("p,scatter", [(4, "auto"), (4, "brick"), (4, "segment"), (6, "auto"), (6, "brick")])
 def test_headline_workload_vs_oracle(cuda, p, scatter):
     """The benchmarked FP64 apply at full size against the oracle at 1e-12: q_data built on the GPU, then
     Apply and ApplyAdd with the library's default E^T (colours at p=4, the multiplicity split at p=6) and
-    with bricks."""
+    with bricks and segments."""
     m, Q, u, qd, v_ref = headline_reference(p)
     print("headline vs oracle p", p, scatter, "dofs", m.num_nodes, flush=True)
     op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q, scatter=scatter)
