@@ -344,12 +344,13 @@ int CeedMi355BatchCGCreate(int32_t num_ops, const CeedMi355Operator *ops_host, c
 /* Solve every system from the given x until ||r_s|| <= rtol ||b_s|| (per
  * system; a converged system is frozen) or max_iter iterations.  Checks every
  * check_every iterations; use_graph captures check_every iterations in a
- * hipGraph.  Per system: iterations run, the true relative residual, and the
+ * hipGraph.  Per system: iterations run, the true relative residual, the
  * index of the check (0 = before the first iteration) at which it was first
- * seen converged (-1 if never). */
+ * seen converged (-1 if never), and (converged_ms_host, may be NULL) the host
+ * time of that check in ms from the call's start (NaN if never). */
 int CeedMi355BatchCGSolve(CeedMi355BatchCG cg, const double *b, double *x, int32_t max_iter, double rtol, int32_t check_every, int use_graph,
                           void *stream, int32_t *iterations_host, double *rel_residual_host, int32_t *converged_at_check_host,
-                          int32_t *num_checks_host);
+                          int32_t *num_checks_host, double *converged_ms_host);
 int CeedMi355BatchCGDestroy(CeedMi355BatchCG *cg);
 
 typedef struct CeedMi355BatchSmall_private *CeedMi355BatchSmall;

@@ -60,7 +60,7 @@ SIGNATURES = {
     "CeedMi355CGSetJacobi": (c_int, [vp, vp, vp]),
     "CeedMi355CGDestroy": (c_int, [C.POINTER(vp)]),
     "CeedMi355BatchCGCreate": (c_int, [c_i32, vp, vp, c_i32, vp, C.POINTER(vp)]),
-    "CeedMi355BatchCGSolve": (c_int, [vp, vp, vp, c_i32, C.c_double, c_i32, c_int, vp, vp, vp, vp, vp]),
+    "CeedMi355BatchCGSolve": (c_int, [vp, vp, vp, c_i32, C.c_double, c_i32, c_int, vp, vp, vp, vp, vp, vp]),
     "CeedMi355BatchCGDestroy": (c_int, [C.POINTER(vp)]),
     "CeedMi355BatchSmallCreate": (c_int, [c_i32, vp, vp, c_i32, vp, C.POINTER(vp)]),
     "CeedMi355BatchSmallSolve": (c_int, [vp, vp, vp, c_i32, C.c_double, vp, vp, vp, vp]),

@@ -140,8 +140,9 @@ class BatchCG:
     def solve(self, b: torch.Tensor, x: torch.Tensor, max_iter: int = 5000, rtol: float = 1e-8, check_every: int = 25,
               use_graph: bool = True) -> dict:
         """Solve every system; returns per system (indexed like ``systems``): iterations, true relative residual,
-        the check at which a group-batched system was first seen converged (-1 for the small systems), and for the
-        small systems the ms from their launch's start to their completion (NaN for the others)."""
+        the check at which a group-batched system was first seen converged (-1 for the small systems), and the ms
+        from the call's start to the system's completion (group-batched: the host time of that check; small systems:
+        the device clock from their launch's start; NaN where a system never converged)."""
         ns, nl = len(self.systems), self.num_large
         it = np.zeros(ns, dtype=np.int32)
         rr = np.zeros(ns)
@@ -170,10 +171,10 @@ class BatchCG:
             th = threading.Thread(target=run_small)
             th.start()
         if self._h is not None:
-            itl, rrl, convl = np.zeros(nl, dtype=np.int32), np.zeros(nl), np.zeros(nl, dtype=np.int32)
+            itl, rrl, convl, msl = np.zeros(nl, dtype=np.int32), np.zeros(nl), np.zeros(nl, dtype=np.int32), np.zeros(nl)
             _lib.call("CeedMi355BatchCGSolve", self._h, b.data_ptr(), x.data_ptr(), int(max_iter), float(rtol), int(check_every),
-                      int(use_graph), ops._stream(), itl.ctypes.data, rrl.ctypes.data, convl.ctypes.data, C.byref(checks))
-            it[:nl], rr[:nl], conv[:nl] = itl, rrl, convl
+                      int(use_graph), ops._stream(), itl.ctypes.data, rrl.ctypes.data, convl.ctypes.data, C.byref(checks), msl.ctypes.data)
+            it[:nl], rr[:nl], conv[:nl], done[:nl] = itl, rrl, convl, msl
         if th is not None:
             th.join()
             if err:

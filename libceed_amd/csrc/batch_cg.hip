@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <utility>
 #include <vector>
 
@@ -357,8 +358,9 @@ extern "C" int CeedMi355BatchCGCreate(int32_t num_ops, const CeedMi355Operator *
 
 extern "C" int CeedMi355BatchCGSolve(CeedMi355BatchCG cg, const double *b, double *x, int32_t max_iter, double rtol, int32_t check_every,
                                      int use_graph, void *stream, int32_t *iterations_host, double *rel_residual_host, int32_t *converged_at_check_host,
-                                     int32_t *num_checks) {
+                                     int32_t *num_checks, double *converged_ms_host) {
   MI355_CHECK(cg && check_every >= 1, CEED_MI355_ERROR_BACKEND, "bad batched CG arguments");
+  const auto t_start = std::chrono::steady_clock::now();
   hipStream_t s  = cg->stream;
   const int   nc = cg->num_chunks;
   int         err;
@@ -374,14 +376,21 @@ extern "C" int CeedMi355BatchCGSolve(CeedMi355BatchCG cg, const double *b, doubl
   MI355_LAUNCH_CHECK();
   if ((err = finish(cg, kRR, 0))) return err;
   if ((err = read_back(cg))) return err;
-  for (int32_t i = 0; i < cg->num_sys; i++) converged_at_check_host[i] = -1;
+  for (int32_t i = 0; i < cg->num_sys; i++) {
+    converged_at_check_host[i] = -1;
+    if (converged_ms_host) converged_ms_host[i] = NAN;
+  }
   int it = 0, checks = 0;
   for (;;) {
-    int active = 0;
+    int          active = 0;
+    const double now_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     for (int32_t i = 0; i < cg->num_sys; i++) {
       const double *sc = cg->h_scal + (int64_t)i * kNumSlots;
       if (sc[kRR] > sc[kTol2]) active++;
-      else if (converged_at_check_host[i] < 0) converged_at_check_host[i] = checks;
+      else if (converged_at_check_host[i] < 0) {
+        converged_at_check_host[i] = checks;
+        if (converged_ms_host) converged_ms_host[i] = now_ms;  // read_back has waited for this check's iterations
+      }
     }
     if (!active || it >= max_iter) break;
     const int len = check_every < max_iter - it ? check_every : max_iter - it;

@@ -41,6 +41,11 @@ def test_batch_converges_and_matches_single_solves(batch):
     st = batch.solve(b, x, max_iter=3000, rtol=1e-10, check_every=10)
     assert np.all(st["rel_residual"] <= 1e-10), st["rel_residual"]
     assert np.all(st["converged_at_check"] >= 0)
+    # completion times: finite, positive, ordered like the checks at which the systems converged
+    done, at = st["done_ms"], st["converged_at_check"]
+    assert np.all(np.isfinite(done)) and np.all(done > 0)
+    order = np.argsort(at, kind="stable")
+    assert np.all(np.diff(done[order]) >= 0)
     for (kind, P, Q), (layout, epb, ms) in batch.tuning.items():
         assert layout in (ops.LAYOUT_COLUMNS, ops.LAYOUT_LINES) and 1 <= epb and epb * Q * Q <= 256 and ms > 0
     for i, s in enumerate(batch.systems):
