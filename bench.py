@@ -544,8 +544,10 @@ def main() -> None:
             print(f"# p={p}: {sweep[str(p)]}", file=sys.stderr, flush=True)
     sizes = {}
     if not args.no_sweep:
-        for label, d, nc in (("p4_2M", 2e6, 1), ("p4_32M", 32e6, 1), ("p4_8M_vector3", 8e6, 3)):
-            r = run_case(4, d, torch.float64, max(10, args.steps // 4), 3, num_comp=nc)
+        # the other sizes of the sweep's workload (2M and 32M DoFs) and the 3-component operator (8M nodes), p = 1..8
+        for p, label, d, nc in [(p, f"p{p}_{tag}", d, nc) for p in range(1, 9) for tag, d, nc in (("2M", 2e6, 1), ("32M", 32e6, 1),
+                                                                                                    ("8M_vector3", 8e6, 3))]:
+            r = run_case(p, d, torch.float64, max(10, args.steps // 4), 3, num_comp=nc)
             sizes[label] = {"dofs": r["dofs"], "num_comp": nc, "fp64_gdofs": round(r["gdofs"], 3), "fp64_hbm_frac": round(r["hbm_frac"], 4),
                             "launch": [r["layout"], r["elems_per_block"]]}
             print(f"# {label}: {sizes[label]}", file=sys.stderr, flush=True)

@@ -33,17 +33,16 @@ class HexMesh:
 
 
 def _lex_offsets(n: int, p: int) -> np.ndarray:
+    """Element offsets [n^3, (p+1)^3] of the lexicographic node numbering: element k = ex + n (ey + n ez) and local node
+    i + P (j + P k), both x fastest, hold global node (ex p + i) + M ((ey p + j) + M (ez p + k)), M = n p + 1."""
     P, M = p + 1, n * p + 1
-    e = np.arange(n)
-    ex, ey, ez = np.meshgrid(e, e, e, indexing="ij")           # [ex, ey, ez]
-    ex, ey, ez = (a.transpose(2, 1, 0).reshape(-1) for a in (ex, ey, ez))  # element index x fastest
-    i = np.arange(P)
-    li, lj, lk = np.meshgrid(i, i, i, indexing="ij")
-    li, lj, lk = (a.transpose(2, 1, 0).reshape(-1) for a in (li, lj, lk))  # local node x fastest
-    gx = ex[:, None] * p + li[None, :]
-    gy = ey[:, None] * p + lj[None, :]
-    gz = ez[:, None] * p + lk[None, :]
-    return (gx + M * (gy + M * gz)).astype(np.int32)
+    k = np.arange(n ** 3, dtype=np.int64)
+    base = p * ((k % n) + M * ((k // n) % n + M * (k // (n * n))))   # global node of each element's local node 0
+    i = np.arange(P, dtype=np.int64)
+    local = (i[None, None, :] + M * (i[None, :, None] + M * i[:, None, None])).reshape(-1)  # [z][y][x], x fastest
+    out = np.empty((n ** 3, P ** 3), dtype=np.int32)
+    np.add(base[:, None], local[None, :], out=out, casting="unsafe")
+    return out
 
 
 def hex_mesh(n: int, p: int, seed: int = 0, perturb: float = 0.1) -> HexMesh:
@@ -53,18 +52,19 @@ def hex_mesh(n: int, p: int, seed: int = 0, perturb: float = 0.1) -> HexMesh:
     nv1 = n + 1
     h = 1.0 / n
     g = np.linspace(0.0, 1.0, nv1)
-    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
-    X, Y, Z = (a.transpose(2, 1, 0).reshape(-1).copy() for a in (X, Y, Z))
+    # vertex ix + nv1 (iy + nv1 iz) at (g[ix], g[iy], g[iz])
+    X, Y, Z = np.tile(g, nv1 * nv1), np.tile(np.repeat(g, nv1), nv1), np.repeat(g, nv1 * nv1)
     rng = np.random.default_rng(seed)
     d = rng.uniform(-1.0, 1.0, size=(3, nv1 ** 3)) * perturb * h
-    idx = np.arange(nv1)
-    iz, iy, ix = np.meshgrid(idx, idx, idx, indexing="ij")
-    interior = ((ix > 0) & (ix < n) & (iy > 0) & (iy < n) & (iz > 0) & (iz < n)).reshape(-1)
+    inner = np.zeros(nv1, dtype=bool)
+    inner[1:n] = True
+    interior = np.tile(inner, nv1 * nv1) & np.tile(np.repeat(inner, nv1), nv1) & np.repeat(inner, nv1 * nv1)
     for a, dd in zip((X, Y, Z), d):
         a[interior] += dd[interior]
     coords = np.concatenate([X, Y, Z]).astype(np.float64)
-    return HexMesh(n=n, p=p, num_elem=n ** 3, num_nodes=(n * p + 1) ** 3, offsets=_lex_offsets(n, p),
-                   mesh_offsets=_lex_offsets(n, 1), coords=coords, num_vertices=nv1 ** 3)
+    offsets = _lex_offsets(n, p)
+    return HexMesh(n=n, p=p, num_elem=n ** 3, num_nodes=(n * p + 1) ** 3, offsets=offsets,
+                   mesh_offsets=offsets if p == 1 else _lex_offsets(n, 1), coords=coords, num_vertices=nv1 ** 3)
 
 
 def node_coordinates(mesh: HexMesh, gll_nodes: np.ndarray) -> np.ndarray:
