@@ -65,8 +65,8 @@ def time_gpu(fn, steps: int, warmup: int):
     return s.elapsed_time(e) / steps, wall
 
 
-LAYOUT_NAMES = ["columns", "lines"]  # the fused kernel's thread layouts, by their index in the library
-KERNEL_OF_LAYOUT = ["hex_apply_kernel", "hex_apply_lines_kernel"]
+LAYOUT_NAMES = ["columns", "lines", "points"]  # the fused kernel's thread layouts, by their index in the library
+KERNEL_OF_LAYOUT = ["hex_apply_kernel", "hex_apply_lines_kernel", "hex_apply_points_kernel"]
 
 
 def run_case(p: int, dofs: float, dtype, steps: int, warmup: int, seed: int = 0, scatter: str = "auto", epb: int = 0, num_comp: int = 1,

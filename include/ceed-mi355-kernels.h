@@ -213,8 +213,10 @@ int CeedMi355OperatorApply(CeedMi355Operator op, const void *u, void *v, void *s
 int CeedMi355OperatorSetLaunch(CeedMi355Operator op, int32_t elems_per_block);
 /* Thread layout of the fused kernel: 0 = z-columns per thread, x and y
  * contractions read point by point from LDS; 1 = whole 1D lines per thread
- * for the x and y contractions (about a third of the LDS traffic).  Both give
- * identical results.  elems_per_block <= 0 picks the default.  No
+ * for the x and y contractions (about a third of the LDS traffic); 2 = a
+ * thread per quadrature point, all three contractions through LDS (Q^3 <=
+ * 256; not for run-time compiled QFunctions).  All give identical results.
+ * elems_per_block <= 0 picks the default.  No
  * counterpart in the reference: its hip-shared/hip-gen backends fix the
  * thread layout per (P, Q) (backends/hip-shared/ceed-hip-shared-basis.c). */
 int CeedMi355OperatorSetLayout(CeedMi355Operator op, int32_t layout, int32_t elems_per_block);

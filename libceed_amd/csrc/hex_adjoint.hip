@@ -451,7 +451,7 @@ template <int P, int Q>
 int launch_adjoint(const CeedMi355Operator_private *op, const double *u, const double *lam, double *v, double *g, hipStream_t s) {
   const int    nt    = Q * Q;
   const int    epb   = op->epb;
-  const bool   lines = op->layout == kLayoutLines;
+  const bool   lines = op->layout == kLayoutLines;  // (plans in the point layout take the column kernel here)
   const size_t lds   = sizeof(double) * (size_t)(lines ? epb * 5 * Q * Q * Q : Q * P + Q * Q + epb * 3 * Q * Q * Q);
   const bool   tmap  = op->scatter == 0;
   MI355_CHECK(op->scatter != CEED_MI355_PLAN_SEGMENT, CEED_MI355_ERROR_UNSUPPORTED,

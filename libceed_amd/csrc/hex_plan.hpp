@@ -9,6 +9,7 @@
 #include "hex_brick.hpp"
 #include "hex_fused.hpp"
 #include "hex_lines.hpp"
+#include "hex_points.hpp"
 #include "mi355_common.h"
 
 struct CeedMi355Operator_private;
@@ -33,9 +34,13 @@ int brick_evec(const CeedMi355Operator_private *op, const void *evec, void *v, i
 ApplyFn pick_apply_f64(int P, int Q, int num_comp, int qf);
 ApplyFn pick_apply_f32(int P, int Q, int num_comp, int qf);
 // Thread layouts of the fused kernel: z-columns with x/y contractions read
-// point by point from LDS (hex_apply_kernel), or whole 1D lines per thread
-// for the x/y contractions (hex_apply_lines_kernel).
-enum { kLayoutColumns = 0, kLayoutLines = 1 };
+// point by point from LDS (hex_apply_kernel), whole 1D lines per thread for
+// the x/y contractions (hex_apply_lines_kernel), or a thread per quadrature
+// point with all three directions through LDS (hex_apply_points_kernel,
+// Q^3 <= 256).
+enum { kLayoutColumns = 0, kLayoutLines = 1, kLayoutPoints = 2 };
+// Threads an element takes in a layout.
+constexpr int layout_threads(int Q, int layout) { return layout == kLayoutPoints ? Q * Q * Q : Q * Q; }
 // LDS a gfx950 workgroup may allocate (the CU's whole 160 KiB), and the
 // budget the default launch keeps to (so that several workgroups share a CU).
 constexpr int kMaxLdsBytes = 160 * 1024, kDefaultLdsBytes = 64 * 1024;
@@ -163,10 +168,12 @@ namespace mi355 {
 
 template <int P, int Q, int NC, int QF, typename T>
 int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v, int overwrite, hipStream_t s) {
-  const int    nt    = Q * Q;
-  const int    epb   = op->epb;
-  const bool   lines = op->layout == kLayoutLines;
-  const size_t lds   = sizeof(T) * (size_t)(lines ? hex_lines_lds_entries<P, Q, QF>(epb) : hex_lds_entries<P, Q, QF>(epb));
+  const int    nt     = Q * Q;
+  const int    epb    = op->epb;
+  const bool   lines  = op->layout == kLayoutLines;
+  const bool   points = op->layout == kLayoutPoints;
+  const size_t lds    = sizeof(T) * (size_t)(lines ? hex_lines_lds_entries<P, Q, QF>(epb)
+                                                   : points ? hex_points_lds_entries<P, Q, QF>(epb) : hex_lds_entries<P, Q, QF>(epb));
   if (op->scatter == CEED_MI355_PLAN_BRICK) {
     if (op->num_bricks == 0) return CEED_MI355_SUCCESS;
     const int ns = op->brick_shape[0] * op->brick_shape[1] * op->brick_shape[2];
@@ -209,7 +216,14 @@ int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v
       hex_apply_lines_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(
           b, e, scatter, op->d_offsets, op->comp_stride, (const T *)op->d_qdata, (const T *)op->d_mats, (const T *)u, target, (T *)op->d_evec,
           op->d_tpos, op->d_elem_sys, op->d_sys_active, nullptr);
-    else
+    else if (points) {
+      if constexpr (Q * Q * Q <= 256)
+        hex_apply_points_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * Q * Q * Q, lds, s>>>(
+            b, e, scatter, op->d_offsets, op->comp_stride, (const T *)op->d_qdata, (const T *)op->d_mats, (const T *)u, target,
+            (T *)op->d_evec, op->d_tpos, op->d_elem_sys, op->d_sys_active);
+      else
+        MI355_CHECK(false, CEED_MI355_ERROR_BACKEND, "the point layout needs Q^3 <= 256 (Q = %d)", Q);
+    } else
       hex_apply_kernel<P, Q, NC, QF, T><<<(unsigned)grid, epb * nt, lds, s>>>(b, e, scatter, op->d_offsets, op->comp_stride, (const T *)op->d_qdata,
                                                                             (const T *)op->d_mats, (const T *)u, target, (T *)op->d_evec,
                                                                             op->d_tpos, op->d_elem_sys, op->d_sys_active);

@@ -17,6 +17,7 @@ void collocated_grad(int P, int Q, const double *interp, const double *grad, dou
 
 int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes, int layout) {
   if (layout == kLayoutLines) return scalar_bytes * epb * (qf == kQfMass ? 2 : 3) * Q * Q * Q;
+  if (layout == kLayoutPoints) return scalar_bytes * (Q * P + Q * Q + epb * (qf == kQfMass ? 3 : 4) * Q * Q * Q);
   const int slabs = qf == kQfMass ? 2 : 3;
   return scalar_bytes * (Q * P + Q * Q + epb * slabs * Q * Q * Q);
 }
@@ -44,7 +45,7 @@ int default_epb(int P, int Q, int qf, int scalar_bytes, int layout) {
     return epb;
   }
   // as many elements as fit 256 threads and the default LDS budget
-  int epb = 256 / (Q * Q);
+  int epb = 256 / layout_threads(Q, layout);
   if (epb < 1) epb = 1;
   while (epb > 1 && lds_bytes(P, Q, qf, epb, scalar_bytes, layout) > kDefaultLdsBytes) epb--;
   return epb;
@@ -250,10 +251,16 @@ bool tuned_launch(const CeedMi355Operator op, int *layout, int *epb) {
   return true;
 }
 
+// A known layout the plan's kernels exist in (run-time compiled QFunctions:
+// the line layout only; the point layout up to Q^3 = 256 threads)
+bool layout_exists(int Q, int qf, int layout) {
+  return layout == mi355::kLayoutLines || (qf != kQfJit && (layout == mi355::kLayoutColumns || (layout == mi355::kLayoutPoints && Q * Q * Q <= 256)));
+}
+
 bool admissible(int P, int Q, int qf, int num_comp, int scalar_bytes, int layout, int epb) {
   (void)num_comp;
-  return ((layout == mi355::kLayoutColumns && qf != kQfJit) || layout == mi355::kLayoutLines) && epb >= 1 &&
-         epb * Q * Q <= mi355::max_threads(layout) && mi355::lds_bytes(P, Q, qf, epb, scalar_bytes, layout) <= mi355::kMaxLdsBytes;
+  return layout_exists(Q, qf, layout) && epb >= 1 && epb * mi355::layout_threads(Q, layout) <= mi355::max_threads(layout) &&
+         mi355::lds_bytes(P, Q, qf, epb, scalar_bytes, layout) <= mi355::kMaxLdsBytes;
 }
 
 // A launch the plan can take: segment plans have one (the line layout with an
@@ -519,8 +526,7 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
   }
   if (const char *env = getenv("CEED_MI355_LAYOUT")) {
     const int l = atoi(env);
-    if ((l == mi355::kLayoutColumns && qf != kQfJit) || l == mi355::kLayoutLines)
-      op->layout = l, op->epb = mi355::default_epb(P_1d, Q_1d, qf, op->scalar_bytes, l);
+    if (layout_exists(Q_1d, qf, l)) op->layout = l, op->epb = mi355::default_epb(P_1d, Q_1d, qf, op->scalar_bytes, l);
   }
   if (qf == kQfJit && op->layout != mi355::kLayoutLines)  // run-time compiled kernels exist in the line layout only
     op->layout = mi355::kLayoutLines, op->epb = mi355::default_epb(P_1d, Q_1d, qf, op->scalar_bytes, op->layout);
@@ -826,9 +832,11 @@ extern "C" int CeedMi355OperatorSetLaunch(CeedMi355Operator op, int32_t elems_pe
 
 extern "C" int CeedMi355OperatorSetLayout(CeedMi355Operator op, int32_t layout, int32_t elems_per_block) {
   MI355_CHECK(op, CEED_MI355_ERROR_MAJOR, "null operator");
-  MI355_CHECK(layout == mi355::kLayoutColumns || layout == mi355::kLayoutLines, CEED_MI355_ERROR_BACKEND, "unknown thread layout %d", layout);
+  MI355_CHECK(layout == mi355::kLayoutColumns || layout == mi355::kLayoutLines || layout == mi355::kLayoutPoints, CEED_MI355_ERROR_BACKEND,
+              "unknown thread layout %d", layout);
   MI355_CHECK(layout == mi355::kLayoutLines || op->qf != kQfJit, CEED_MI355_ERROR_UNSUPPORTED,
               "plans of run-time compiled QFunctions have the line layout only");
+  MI355_CHECK(layout_exists(op->Q, op->qf, layout), CEED_MI355_ERROR_UNSUPPORTED, "the point layout needs Q^3 <= 256 (Q = %d)", op->Q);
   const int epb = elems_per_block > 0                       ? elems_per_block
                   : op->scatter == CEED_MI355_PLAN_SEGMENT ? op->seg_max
                                                            : mi355::default_epb(op->P, op->Q, op->qf, op->scalar_bytes, layout);
@@ -868,8 +876,8 @@ extern "C" int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int
   int       best = start, best_layout = start_layout;
   float     bms  = 1e30f;
   int       err  = CEED_MI355_SUCCESS;
-  for (int layout = mi355::kLayoutColumns; layout <= mi355::kLayoutLines && !err; layout++) {
-    for (int epb = 1; epb * op->Q * op->Q <= mi355::max_threads(layout) && !err; epb++) {
+  for (int layout = mi355::kLayoutColumns; layout <= mi355::kLayoutPoints && !err; layout++) {
+    for (int epb = 1; epb * mi355::layout_threads(op->Q, layout) <= mi355::max_threads(layout) && !err; epb++) {
       if (!plan_admissible(op, layout, epb)) continue;
       op->layout = layout;
       op->epb    = epb;

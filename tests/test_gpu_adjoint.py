@@ -31,7 +31,7 @@ def problem(cuda):
     return m, P, Q, qd, kappa
 
 
-@pytest.mark.parametrize("layout", [ops.LAYOUT_LINES, ops.LAYOUT_COLUMNS])
+@pytest.mark.parametrize("layout", [ops.LAYOUT_LINES, ops.LAYOUT_COLUMNS, ops.LAYOUT_POINTS])
 @pytest.mark.parametrize("scatter", ["tmap", "color", "split", "atomic"])
 def test_forward_and_adjoint_vs_oracle(problem, scatter, layout):
     m, P, Q, qd, kappa = problem

@@ -47,7 +47,8 @@ def test_batch_converges_and_matches_single_solves(batch):
     order = np.argsort(at, kind="stable")
     assert np.all(np.diff(done[order]) >= 0)
     for (kind, P, Q), (layout, epb, ms) in batch.tuning.items():
-        assert layout in (ops.LAYOUT_COLUMNS, ops.LAYOUT_LINES) and 1 <= epb and epb * Q * Q <= 256 and ms > 0
+        assert layout in (ops.LAYOUT_COLUMNS, ops.LAYOUT_LINES, ops.LAYOUT_POINTS) and ms > 0
+        assert 1 <= epb and epb * (Q ** 3 if layout == ops.LAYOUT_POINTS else Q * Q) <= 256
     for i, s in enumerate(batch.systems):
         r = batch.range(i)
         op = ops.HexOperator.from_mesh(s.kind, s.mesh, Q=s.mesh.P + 1)

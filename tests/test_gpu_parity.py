@@ -187,9 +187,9 @@ def test_operator_vs_oracle(cuda, kind, nc, n, p, Q):
                                                  (0, 3, 3, 2, 3, False), (1, 1, 5, 1, 2, False), (1, 1, 3, 5, 6, True), (0, 1, 3, 4, 5, True),
                                                  (2, 1, 3, 4, 5, False), (2, 1, 2, 6, 8, False)])
 def test_thread_layouts_identical(cuda, kind, nc, n, p, Q, fp32):
-    """Both thread layouts of the fused kernel (z-columns, 1D lines), every admissible elements-per-workgroup count
-    and every deterministic E^T mode: bitwise identical results across the launches (so the autotuner's choice
-    never changes them) and oracle parity."""
+    """Every thread layout of the fused kernel (z-columns, 1D lines, and a thread per point where Q^3 <= 256), every
+    admissible elements-per-workgroup count and every deterministic E^T mode: bitwise identical results across the
+    launches (so the autotuner's choice never changes them) and oracle parity."""
     from libceed_amd.mesh import quadrature_points, smooth_random_field
 
     m = hex_mesh(n, p, seed=3 * n + p)
@@ -207,8 +207,8 @@ def test_thread_layouts_identical(cuda, kind, nc, n, p, Q, fp32):
         if kind == ops.QF_DIFFUSION_KAPPA:
             op.set_kappa(dev(kappa))
         outs = []
-        for layout in (ops.LAYOUT_COLUMNS, ops.LAYOUT_LINES):
-            for epb in range(1, 256 // (Q * Q) + 1):
+        for layout in (ops.LAYOUT_COLUMNS, ops.LAYOUT_LINES, ops.LAYOUT_POINTS):
+            for epb in range(1, 256 // (Q ** 3 if layout == ops.LAYOUT_POINTS else Q * Q) + 1):
                 try:
                     op.set_layout(layout, epb)
                 except ops.CeedMi355Error:
@@ -217,7 +217,7 @@ def test_thread_layouts_identical(cuda, kind, nc, n, p, Q, fp32):
                 v = torch.full((nc * m.num_nodes,), 3.0, dtype=dtype, device="cuda")
                 op.apply(dev(u, dtype), v)
                 outs.append((layout, epb, host(v)))
-        assert {o[0] for o in outs} == {ops.LAYOUT_COLUMNS, ops.LAYOUT_LINES}
+        assert {o[0] for o in outs} == ({ops.LAYOUT_COLUMNS, ops.LAYOUT_LINES} | ({ops.LAYOUT_POINTS} if Q ** 3 <= 256 else set()))
         for layout, epb, o in outs[1:]:
             assert np.array_equal(o, outs[0][2]), (det, layout, epb)
         assert rel_l2(outs[0][2], v_ref) < (RTOL32 if fp32 else RTOL64), det
