@@ -589,16 +589,16 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
     int max_len = 256 / (Q_1d * Q_1d);
     while (max_len > 1 && mi355::lds_bytes(P_1d, Q_1d, qf, max_len, op->scalar_bytes, mi355::kLayoutLines) > mi355::kMaxLdsBytes) max_len--;
     mi355::SegmentLayout L;
-    ierr = max_len >= 1 ? mi355::segment_layout(offsets_host, num_elem, P_1d, max_len, L) : CEED_MI355_ERROR_UNSUPPORTED;
+    ierr = mi355::segment_layout(offsets_host, num_elem, P_1d, max_len, L);
     if (ierr) {
       delete op;
       return ierr;
     }
-    op->perm    = std::move(L.perm);
+    op->perm      = std::move(L.perm);
     op->seg_color = std::move(L.color);
-    op->seg_max = std::max(1, L.max_len);
-    op->layout  = mi355::kLayoutLines;
-    op->epb     = op->seg_max;
+    op->seg_max   = std::max(1, L.max_len);  // (1 for a plan without elements)
+    op->layout    = mi355::kLayoutLines;
+    op->epb       = op->seg_max;
     op->color_start.assign(op->seg_color.size(), 0);
     for (size_t c = 0; c < op->seg_color.size(); c++) op->color_start[c] = L.start[op->seg_color[c]];
     seg_first.assign((size_t)num_elem, 0);
