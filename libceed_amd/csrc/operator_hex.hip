@@ -471,7 +471,8 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
   op->l_size = l_size;
   op->apply = fn;
   // E^T mode; by default colours only where they are faster than the
-  // transpose map on MI355X: below P = 6 in FP64 (P = 7 in FP32), where the
+  // transpose map on MI355X: below P = 6 (FP64 and FP32; FP32 at P = 6, 8M
+  // DoFs: map 0.169 ms, colours 0.200), where the
   // E-vector round trip of the map costs more than the colour launches, and
   // from 10M E-vector entries, where each of the (about 8) colour launches
   // still fills the GPU (at 2M DoFs, p = 4, the map is 1.33x faster; at 8M,
@@ -487,7 +488,7 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
   else if (flags & CEED_MI355_PLAN_COLOR) op->scatter = CEED_MI355_PLAN_COLOR;
   else if (flags & CEED_MI355_PLAN_TMAP) op->scatter = 0;
   else if (big && scalar_type == CEED_MI355_FP64 && (P_1d == 7 || P_1d == 8)) op->scatter = CEED_MI355_PLAN_SPLIT;
-  else op->scatter = P_1d >= (scalar_type == CEED_MI355_FP32 ? 7 : 6) || !big ? 0 : CEED_MI355_PLAN_COLOR;
+  else op->scatter = P_1d >= 6 || !big ? 0 : CEED_MI355_PLAN_COLOR;
   const int explicit_modes =
       CEED_MI355_PLAN_ATOMIC | CEED_MI355_PLAN_COLOR | CEED_MI355_PLAN_TMAP | CEED_MI355_PLAN_SPLIT | CEED_MI355_PLAN_BRICK | CEED_MI355_PLAN_SEGMENT;
   const int default_scatter = op->scatter;

@@ -21,7 +21,7 @@ EVAL_NONE, EVAL_INTERP, EVAL_GRAD, EVAL_WEIGHT = 0, 1, 2, 16
 QF_MASS, QF_DIFFUSION, QF_DIFFUSION_KAPPA = 0, 1, 2
 LAYOUT_COLUMNS, LAYOUT_LINES, LAYOUT_POINTS = 0, 1, 2  # thread layouts of the fused kernel
 FP64, FP32 = 0, 1
-# E^T of the fused plan: "auto" (colours below P = 6 in FP64 / P = 7 in FP32 and the multiplicity split at P = 7, 8 in
+# E^T of the fused plan: "auto" (colours below P = 6 and the multiplicity split at P = 7, 8 in
 # FP64, both on meshes of at least 10M E-vector entries; the transpose map otherwise), transpose offset map, element
 # colours, multiplicity split (atomic adds for entries of at most two contributions, the ordered map for the rest),
 # hardware atomics, bricks of face-sharing elements, coloured x-segments of face-sharing elements
