@@ -594,6 +594,15 @@ __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin
       }
       __syncthreads();
       // ---- pointwise (Poisson3DApply, times kappa for the kappa-weighted operator) on the column
+      // From Q = 10 the column's z-derivatives wait in S1 (free until D_x^T
+      // below; only this thread's entries) rather than in registers beside
+      // the late q_data loads (interleaved A/B on MI355X, 8M DoFs: p=8 0.283
+      // vs 0.293 ms; at Q = 9, p=7, 0.280 vs 0.274, so not there).
+      constexpr bool kDzLds = !kQdEarly && Q >= 10;
+      if constexpr (kDzLds) {
+#pragma unroll
+        for (int qz = 0; qz < Q; qz++) S1[at(qz, cy, cx)] = dz[qz];
+      }
       if constexpr (!kQdEarly) load_q();
       T vz[Q];
 #pragma unroll
@@ -611,7 +620,7 @@ __global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin
           }
           poisson_regs<T>(s6, S2[at(qz, cy, cx)], S0[at(qz, cy, cx)], dz[qz], vx, vy, vzz);
         } else {
-          poisson_regs<T>(sq[qz], S2[at(qz, cy, cx)], S0[at(qz, cy, cx)], dz[qz], vx, vy, vzz);
+          poisson_regs<T>(sq[qz], S2[at(qz, cy, cx)], S0[at(qz, cy, cx)], kDzLds ? S1[at(qz, cy, cx)] : dz[qz], vx, vy, vzz);
         }
         if constexpr (QF == kQfDiffusionKappa) {
           vx *= sq[qz][6];
