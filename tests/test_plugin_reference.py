@@ -16,7 +16,11 @@ oracle's, so the whole libCEED call sequence runs up to the device boundary:
     gallery QFunctions and check their own volumes and surface areas.
 
 Also: the same examples compile and link unchanged against this repository's
-include/ and libceed_amd/lib/libceed-mi355.so (no GPU needed to link)."""
+include/ and libceed_amd/lib/libceed-mi355.so (no GPU needed to link).
+
+MI355_SANITIZE=1 builds the backend and the programs with AddressSanitizer and
+UndefinedBehaviorSanitizer (leak detection on), so the same tests check the
+backend's host code for memory errors and leaks."""
 
 import os
 import shutil
@@ -54,7 +58,8 @@ def plugin(tmp_path_factory):
         shutil.copy(ROOT / "include" / h, inc / h)
     so = d / "libmi355-plugin.so"
     rpath = f"-Wl,-rpath,{REF_LIB.parent}:{ORACLE_LIB.parent}:{d}"
-    run(["gcc", "-shared", "-fPIC", "-O1", "-std=c11", "-Wall", "-Werror", f"-I{REF / 'include'}", f"-I{inc}", f"-I{HOST}", f"-I{ROOT / 'oracle'}",
+    san = ["-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-fno-sanitize-recover=undefined"] if os.environ.get("MI355_SANITIZE") else []
+    run(["gcc", "-shared", "-fPIC", "-O1", *san, "-std=c11", "-Wall", "-Werror", f"-I{REF / 'include'}", f"-I{inc}", f"-I{HOST}", f"-I{ROOT / 'oracle'}",
          str(HOST / "ceed-mi355-objects.c"), str(HOST / "ceed-mi355-operator.c"), str(PLUGIN / "mi355-plugin.c"), str(PLUGIN / "mi355-cpu-standin.c"),
          f"-L{REF_LIB.parent}", "-lceed", f"-L{ORACLE_LIB.parent}", "-loracle", "-ldl", "-lm", rpath, "-o", str(so)])
 
@@ -62,7 +67,7 @@ def plugin(tmp_path_factory):
         exe = d / name
         # --no-as-needed: nothing in the program names a plugin symbol, the
         # library is there for its constructor
-        run(["gcc", "-O1", "-std=c11", f"-I{REF / 'include'}", str(src), f"-L{REF_LIB.parent}", "-lceed", "-Wl,--no-as-needed", str(so),
+        run(["gcc", "-O1", *san, "-std=c11", f"-I{REF / 'include'}", str(src), f"-L{REF_LIB.parent}", "-lceed", "-Wl,--no-as-needed", str(so),
              "-Wl,--as-needed", "-lm", rpath, "-o", str(exe)])
         return exe
 
