@@ -120,7 +120,7 @@ typedef struct CeedMi355Operator_private *CeedMi355Operator;
 /* Plan for a 3D tensor hex operator with a gallery pointwise D, applied by
  * one fused kernel launch.  E^T (flags; 0 picks the mode measured fastest on
  * MI355X for the order, precision and size: COLOR for P_1d <= 5 from 10M
- * E-vector entries, SPLIT for P_1d = 7, 8 in FP64 from
+ * E-vector entries, SPLIT for P_1d = 7, 8 in FP64 with one component from
  * 10M entries, TMAP otherwise; CEED_MI355_SCATTER=color|tmap|split|atomic|brick|segment
  * replaces the choice):
  *   CEED_MI355_PLAN_TMAP   the fused kernel stores every contribution into

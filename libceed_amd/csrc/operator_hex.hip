@@ -477,8 +477,10 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
   // from 10M E-vector entries, where each of the (about 8) colour launches
   // still fills the GPU (at 2M DoFs, p = 4, the map is 1.33x faster; at 8M,
   // colours are 1.07x faster).  From 10M entries at P = 7 and 8 in FP64 the
-  // multiplicity split (same Apply bits as the map) is faster still: 1.08x
-  // at p = 6, 1.03x at p = 7, 8M DoFs.
+  // multiplicity split (same Apply bits as the map) is faster still for one
+  // component: 1.08x at p = 6, 1.03x at p = 7, 8M DoFs; with three
+  // components the map is (8M nodes: p = 6 0.691 vs 0.803 ms, p = 7 0.727
+  // vs 0.831 ms).
   const int64_t evec_entries = (int64_t)num_elem * P_1d * P_1d * P_1d * num_comp;
   const bool    big          = evec_entries >= 10000000;
   if (flags & CEED_MI355_PLAN_SEGMENT) op->scatter = CEED_MI355_PLAN_SEGMENT;
@@ -487,7 +489,7 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
   else if (flags & CEED_MI355_PLAN_ATOMIC) op->scatter = CEED_MI355_PLAN_ATOMIC;
   else if (flags & CEED_MI355_PLAN_COLOR) op->scatter = CEED_MI355_PLAN_COLOR;
   else if (flags & CEED_MI355_PLAN_TMAP) op->scatter = 0;
-  else if (big && scalar_type == CEED_MI355_FP64 && (P_1d == 7 || P_1d == 8)) op->scatter = CEED_MI355_PLAN_SPLIT;
+  else if (big && scalar_type == CEED_MI355_FP64 && num_comp == 1 && (P_1d == 7 || P_1d == 8)) op->scatter = CEED_MI355_PLAN_SPLIT;
   else op->scatter = P_1d >= 6 || !big ? 0 : CEED_MI355_PLAN_COLOR;
   const int explicit_modes =
       CEED_MI355_PLAN_ATOMIC | CEED_MI355_PLAN_COLOR | CEED_MI355_PLAN_TMAP | CEED_MI355_PLAN_SPLIT | CEED_MI355_PLAN_BRICK | CEED_MI355_PLAN_SEGMENT;

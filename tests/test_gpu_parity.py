@@ -371,6 +371,19 @@ def test_operator_invariants_large(cuda, p):
     assert abs(torch.dot(a, Ab).item() - torch.dot(b, Aa).item()) < 1e-10 * abs(torch.dot(a, Ab).item())
 
 
+@pytest.mark.parametrize("p,num_comp,mode", [(4, 1, "color"), (5, 1, "tmap"), (6, 1, "split"), (7, 1, "split"), (8, 1, "tmap"),
+                                              (4, 3, "color"), (6, 3, "tmap"), (7, 3, "tmap")])
+def test_default_mode_choice(cuda, p, num_comp, mode):
+    """The default E^T of FP64 diffusion plans with more than 10M E-vector entries, by order and components (the
+    multiplicity split only for one component at P = 7, 8; three components keep the transpose map there)."""
+    from libceed_amd.mesh import mesh_size_for_dofs
+
+    m = hex_mesh(mesh_size_for_dofs(p, 12e6 / num_comp), p, seed=3)
+    assert m.num_elem * m.P ** 3 * num_comp >= 10_000_000
+    A = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=p + 2, num_comp=num_comp, build=False)
+    assert A.scatter_mode() == mode
+
+
 @pytest.mark.parametrize("p,mode", [(4, "color"), (6, "split"), (7, "split")])
 def test_default_modes_full_size(cuda, p, mode):
     """At ~8M DoFs the size-based default E^T is colours (p <= 4) or the multiplicity split (p = 6, 7): the
