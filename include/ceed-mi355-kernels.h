@@ -146,6 +146,7 @@ typedef struct CeedMi355Operator_private *CeedMi355Operator;
  *                          brick-surface nodes finished by an ordered gather
  *                          (mass and diffusion only);
  *   CEED_MI355_PLAN_SEGMENT a workgroup per segment of up to 256 / Q_1d^2
+ *                          (or CEED_MI355_SEGMENT_MAX, if smaller)
  *                          elements chained along x through shared faces;
  *                          the face between consecutive elements is summed in
  *                          LDS (left element's value first), segments are

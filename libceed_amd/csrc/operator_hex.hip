@@ -590,6 +590,7 @@ int mi355::create_plan(int qf, int scalar_type, int32_t num_comp, int32_t num_el
     // a workgroup per segment in the line layout, as many element slots as
     // 256 threads hold (within the CU's LDS), no other launch
     int max_len = 256 / (Q_1d * Q_1d);
+    if (const char *env = getenv("CEED_MI355_SEGMENT_MAX")) max_len = std::max(1, std::min(max_len, atoi(env)));
     while (max_len > 1 && mi355::lds_bytes(P_1d, Q_1d, qf, max_len, op->scalar_bytes, mi355::kLayoutLines) > mi355::kMaxLdsBytes) max_len--;
     mi355::SegmentLayout L;
     ierr = mi355::segment_layout(offsets_host, num_elem, P_1d, max_len, L);
