@@ -406,8 +406,18 @@ __device__ __forceinline__ void line_element(T *S0, T *S1, T *S2, int t, bool ac
 // kSeg: the workgroup runs one segment of elements chained along x
 // (seg_start), summing the face between consecutive elements in LDS before
 // E^T; a template parameter so that the other modes keep their registers.
+// Waves per SIMD the register allocation must leave room for (1: no bound).
+// Only where the bound costs no spills and measured faster: FP64 3-component
+// diffusion at P = 4, Q = 5 fits 4 waves in 115 VGPRs instead of 3 in 129
+// (MI355X, 8M nodes: 0.779-0.789 vs 0.826-0.830 ms per apply).  Measured and
+// not kept: 3-component FP32 at P = 6, Q = 7 (5 waves, no spills) 4% slower;
+// FP64 3-component at P = 5, Q = 6 (4 waves, 8 VGPRs spilled) 7% slower.
+template <int P, int Q, int NCOMP, int QF, typename T>
+constexpr int lines_min_waves() {
+  return sizeof(T) == 8 && QF == kQfDiffusion && NCOMP == 3 && P == 4 && Q == 5 ? 4 : 1;
+}
 template <int P, int Q, int NCOMP, int QF, typename T, bool kSeg = false>
-__global__ void __launch_bounds__(256) hex_apply_lines_kernel(int64_t elem_begin, int64_t elem_end, int scatter,
+__global__ void __launch_bounds__(256, (lines_min_waves<P, Q, NCOMP, QF, T>())) hex_apply_lines_kernel(int64_t elem_begin, int64_t elem_end, int scatter,
                                                                const int32_t *__restrict__ offsets, int32_t comp_stride,
                                                                const T *__restrict__ q_data, const T *__restrict__ mats,
                                                                const T *__restrict__ u, T *__restrict__ v, T *__restrict__ evec,
