@@ -504,3 +504,26 @@ def test_headline_workload_fp32_vs_oracle(cuda):
     v = torch.zeros(m.num_nodes, dtype=torch.float32, device="cuda")
     op.apply(dev(u, torch.float32), v)
     assert rel_l2(host(v), v_ref) < RTOL32
+
+
+@pytest.mark.parametrize("p,n,mode", [(3, 38, "color"), (6, 22, "tmap")])
+def test_vector_default_vs_oracle(cuda, p, n, mode):
+    """The 3-component diffusion operator with the library's default E^T at a size where the size-based default
+    applies (more than 10M E-vector entries; p=3 colours, p=6 the transpose map): q_data built on the GPU, then
+    Apply and ApplyAdd against the oracle at 1e-12."""
+    m = hex_mesh(n, p, seed=43)
+    Q = p + 2
+    assert m.num_elem * m.P ** 3 * 3 >= 10_000_000
+    u = np.random.default_rng(p + 100).uniform(-1, 1, 3 * m.num_nodes)
+    qd = oracle.hex_build_qdata(1, m, Q)
+    v_ref = oracle.hex_apply(1, 3, m.num_elem, p + 1, Q, m.offsets, m.num_nodes, qd, u)
+    print("vector vs oracle p", p, "dofs", 3 * m.num_nodes, flush=True)
+    op = ops.HexOperator.from_mesh(ops.QF_DIFFUSION, m, Q=Q, num_comp=3)
+    assert op.scatter_mode() == mode
+    assert rel_l2(host(op.get_qdata()), qd) < RTOL64
+    v = torch.full((3 * m.num_nodes,), 5.0, dtype=torch.float64, device="cuda")
+    ud = dev(u)
+    op.apply(ud, v)
+    assert rel_l2(host(v), v_ref) < RTOL64
+    op.apply_add(ud, v)
+    assert rel_l2(host(v), 2.0 * v_ref) < RTOL64
