@@ -21,7 +21,8 @@
 //   x-line of nodes (y, z) = (t % P, t / P), t < P*P     B_x, B_x^T
 //   y-line (qx, z) = (t % Q, t / Q), t < Q*P              B_y, B_y^T
 //   x-line (qy, qz) and y-line (qx, qz) = (t % Q, t / Q)  the collocated derivatives
-// Slabs are [z][y][x] with strides (Q*Q, Q, 1); mass needs two per element,
+// Slabs are [z][y][x] with strides (Q*Q, Q, 1), padded at Q = 8 against LDS
+// bank conflicts (line_row_stride, line_plane_stride); mass needs two per element,
 // diffusion three (the y-derivative transpose waits in registers for a barrier
 // and then takes the slab the x-derivative transpose has read: one barrier more
 // than four slabs, and a quarter less LDS, so more workgroups fit a CU).
@@ -67,9 +68,23 @@ struct LineSlabs {
   static constexpr int kSlabs = QF == kQfMass ? 2 : 3;
 };
 
-template <int P, int Q, int QF>
+// Row and plane strides (entries) of the line kernel's slabs, for entries of
+// sb bytes.  At Q = 8 the natural strides (8, 64) are powers of two: the
+// entries an x-line, y-line or x-node-line phase touches in one wavefront
+// then fall into a few LDS bank groups (SQ_LDS_BANK_CONFLICT 32% of the
+// cycles at p=6, 39% with 3 components).  Rows of 9 and planes of 72 (FP64),
+// planes of 73 (FP32) spread them (a bank model of the phases' access
+// patterns: 2.1x fewer conflict cycles in FP64, 3.8x in FP32).  Measured on
+// MI355X, 8M DoFs, ms per apply: FP32 p=6 0.136-0.139 vs 0.148, 3-component
+// FP64 p=6 0.648-0.650 vs 0.689-0.691, scalar FP64 p=6 unchanged within run
+// to run noise; the same search gave no measurable gain at Q = 4 or 10.
+__host__ __device__ constexpr int line_row_stride(int Q, int sb) { return sb == 8 && Q == 8 ? 9 : Q; }
+__host__ __device__ constexpr int line_plane_stride(int Q, int sb) { return Q == 8 ? (sb == 8 ? 72 : 73) : Q * Q; }
+__host__ __device__ constexpr int line_slab_entries(int Q, int sb) { return Q * line_plane_stride(Q, sb); }
+
+template <int P, int Q, int QF, typename T>
 __host__ __device__ constexpr int hex_lines_lds_entries(int epb) {
-  return epb * LineSlabs<QF>::kSlabs * Q * Q * Q;
+  return epb * LineSlabs<QF>::kSlabs * line_slab_entries(Q, (int)sizeof(T));
 }
 
 // The column's geometric factors (QC values at each of its Q points) and how
@@ -424,7 +439,8 @@ __global__ void __launch_bounds__(256, (lines_min_waves<P, Q, NCOMP, QF, T>())) 
                                                                const int32_t *__restrict__ tpos,
                                                                const int32_t *__restrict__ elem_sys, const int32_t *__restrict__ sys_active,
                                                                const int32_t *__restrict__ seg_start) {
-  constexpr int NT = Q * Q, SLAB = Q * Q * Q, P3 = P * P * P, Q3 = Q * Q * Q, QC = QfTraits<QF>::kComps;
+  constexpr int NT = Q * Q, P3 = P * P * P, Q3 = Q * Q * Q, QC = QfTraits<QF>::kComps;
+  constexpr int SR = line_row_stride(Q, sizeof(T)), SP = line_plane_stride(Q, sizeof(T)), SLAB = Q * SP;
   constexpr int NS = LineSlabs<QF>::kSlabs;
   // from Q = 7 one phase's matrix entries no longer fit the SGPRs: load them
   // per output row (or column) instead of per phase
@@ -437,7 +453,7 @@ __global__ void __launch_bounds__(256, (lines_min_waves<P, Q, NCOMP, QF, T>())) 
   T        *S0 = reinterpret_cast<T *>(lds_raw) + (size_t)le * NS * SLAB;
   T        *S1 = S0 + SLAB;
   T        *S2 = S1 + SLAB;  // diffusion only
-  auto      at = [](int z, int y, int x) { return (z * Q + y) * Q + x; };
+  auto      at = [](int z, int y, int x) { return z * SP + y * SR + x; };
 
   // the workgroup's elements: epb consecutive packed elements, or (kSeg)
   // segment elem_begin + chunk, whose consecutive elements share x-faces

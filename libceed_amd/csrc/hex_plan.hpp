@@ -172,7 +172,7 @@ int launch_hex_apply(const CeedMi355Operator_private *op, const void *u, void *v
   const int    epb    = op->epb;
   const bool   lines  = op->layout == kLayoutLines;
   const bool   points = op->layout == kLayoutPoints;
-  const size_t lds    = sizeof(T) * (size_t)(lines ? hex_lines_lds_entries<P, Q, QF>(epb)
+  const size_t lds    = sizeof(T) * (size_t)(lines ? hex_lines_lds_entries<P, Q, QF, T>(epb)
                                                    : points ? hex_points_lds_entries<P, Q, QF>(epb) : hex_lds_entries<P, Q, QF>(epb));
   if (op->scatter == CEED_MI355_PLAN_BRICK) {
     if (op->num_bricks == 0) return CEED_MI355_SUCCESS;

@@ -16,7 +16,7 @@ namespace mi355 {
 void collocated_grad(int P, int Q, const double *interp, const double *grad, double *collo);
 
 int lds_bytes(int P, int Q, int qf, int epb, int scalar_bytes, int layout) {
-  if (layout == kLayoutLines) return scalar_bytes * epb * (qf == kQfMass ? 2 : 3) * Q * Q * Q;
+  if (layout == kLayoutLines) return scalar_bytes * epb * (qf == kQfMass ? 2 : 3) * line_slab_entries(Q, scalar_bytes);
   if (layout == kLayoutPoints) return scalar_bytes * (Q * P + Q * Q + epb * (qf == kQfMass ? 3 : 4) * Q * Q * Q);
   const int slabs = qf == kQfMass ? 2 : 3;
   return scalar_bytes * (Q * P + Q * Q + epb * slabs * Q * Q * Q);
