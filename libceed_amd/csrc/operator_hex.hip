@@ -880,23 +880,47 @@ extern "C" int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int
   int       best = start, best_layout = start_layout;
   float     bms  = 1e30f;
   int       err  = CEED_MI355_SUCCESS;
+  // ms per apply of one launch shape: a warm-up apply, then n timed applies
+  auto time_launch = [&](int layout, int epb, int n, float *ms) {
+    op->layout = layout;
+    op->epb    = epb;
+    int e      = CeedMi355OperatorApply(op, u, v, s);
+    if (e) return e;
+    hipEventRecord(t0, s);
+    for (int r = 0; r < n && !e; r++) e = CeedMi355OperatorApply(op, u, v, s);
+    hipEventRecord(t1, s);
+    hipEventSynchronize(t1);
+    hipEventElapsedTime(ms, t0, t1);
+    *ms /= n;
+    return e;
+  };
+  struct Cand {
+    float ms;
+    int   layout, epb;
+  };
+  std::vector<Cand> cands;
   for (int layout = mi355::kLayoutColumns; layout <= mi355::kLayoutPoints && !err; layout++) {
     for (int epb = 1; epb * mi355::layout_threads(op->Q, layout) <= mi355::max_threads(layout) && !err; epb++) {
       if (!plan_admissible(op, layout, epb)) continue;
-      op->layout = layout;
-      op->epb    = epb;
-      err        = CeedMi355OperatorApply(op, u, v, s);  // warm-up
-      if (err) break;
-      hipEventRecord(t0, s);
-      for (int r = 0; r < reps && !err; r++) err = CeedMi355OperatorApply(op, u, v, s);
-      hipEventRecord(t1, s);
-      hipEventSynchronize(t1);
       float ms = 0;
-      hipEventElapsedTime(&ms, t0, t1);
-      ms /= reps;
-      if (ms < bms) bms = ms, best = epb, best_layout = layout;
+      err = time_launch(layout, epb, reps, &ms);
+      if (!err) cands.push_back({ms, layout, epb});
     }
   }
+  // launches within a few per cent of each other swap places from one timing
+  // to the next: the three fastest are timed again, twice each, interleaved,
+  // with twice the applies, and each keeps its best time
+  std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+  if (cands.size() > 3) cands.resize(3);
+  for (int round = 0; round < 2 && cands.size() > 1 && !err; round++)
+    for (Cand &c : cands) {
+      float ms = 0;
+      err = time_launch(c.layout, c.epb, 2 * reps, &ms);
+      if (err) break;
+      c.ms = std::min(c.ms, ms);
+    }
+  for (const Cand &c : cands)
+    if (c.ms < bms) bms = c.ms, best = c.epb, best_layout = c.layout;
   op->epb    = err ? start : best;
   op->layout = err ? start_layout : best_layout;
   hipEventDestroy(t0);
