@@ -343,7 +343,7 @@ extern "C" int CeedMi355BasisApply(CeedMi355Basis b, int32_t num_elem, int t_mod
 
 extern "C" int CeedMi355BasisDestroy(CeedMi355Basis *basis) {
   if (!basis || !*basis) return CEED_MI355_SUCCESS;
-  hipFree((*basis)->d_mats);
+  (void)hipFree((*basis)->d_mats);
   free(*basis);
   *basis = nullptr;
   return CEED_MI355_SUCCESS;

@@ -225,7 +225,7 @@ int segment(CeedMi355BatchCG cg, double *x, int len) {
 int run_segment(CeedMi355BatchCG cg, const double *b, double *x, int len, bool use_graph) {
   if (!use_graph) return segment(cg, x, len);
   if (!cg->graph || cg->graph_len != len || cg->graph_b != b || cg->graph_x != x) {
-    if (cg->graph) hipGraphExecDestroy(cg->graph);
+    if (cg->graph) (void)hipGraphExecDestroy(cg->graph);
     cg->graph = nullptr;
     hipGraph_t g;
     MI355_HIP(hipStreamBeginCapture(cg->stream, hipStreamCaptureModeThreadLocal));
@@ -234,7 +234,7 @@ int run_segment(CeedMi355BatchCG cg, const double *b, double *x, int len, bool u
     if (err) return err;
     MI355_HIP(e);
     e = hipGraphInstantiate(&cg->graph, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     MI355_HIP(e);
     cg->graph_len = len;
     cg->graph_b   = b;
@@ -419,26 +419,26 @@ extern "C" int CeedMi355BatchCGSolve(CeedMi355BatchCG cg, const double *b, doubl
 extern "C" int CeedMi355BatchCGDestroy(CeedMi355BatchCG *cg) {
   if (!cg || !*cg) return CEED_MI355_SUCCESS;
   CeedMi355BatchCG c = *cg;
-  if (c->stream) hipStreamSynchronize(c->stream);
-  if (c->graph) hipGraphExecDestroy(c->graph);
-  hipFree(c->d_chunk_begin);
-  hipFree(c->d_chunk_sys);
-  hipFree(c->d_sys_chunk);
-  hipFree(c->d_iters);
-  hipHostFree(c->h_iters);
-  hipFree(c->d_scal);
-  hipHostFree(c->h_scal);
-  hipFree(c->d_partial);
-  hipFree(c->r);
-  hipFree(c->p);
-  hipFree(c->Ap);
-  hipFree(c->d_active);
-  hipFree(c->d_prev);
-  for (int32_t *d : c->d_elem_sys) hipFree(d);
-  for (int64_t *d : c->d_op_chunk_begin) hipFree(d);
-  for (int32_t *d : c->d_op_chunk_sys) hipFree(d);
-  if (c->ev) hipEventDestroy(c->ev);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  (void)hipFree(c->d_chunk_begin);
+  (void)hipFree(c->d_chunk_sys);
+  (void)hipFree(c->d_sys_chunk);
+  (void)hipFree(c->d_iters);
+  (void)hipHostFree(c->h_iters);
+  (void)hipFree(c->d_scal);
+  (void)hipHostFree(c->h_scal);
+  (void)hipFree(c->d_partial);
+  (void)hipFree(c->r);
+  (void)hipFree(c->p);
+  (void)hipFree(c->Ap);
+  (void)hipFree(c->d_active);
+  (void)hipFree(c->d_prev);
+  for (int32_t *d : c->d_elem_sys) (void)hipFree(d);
+  for (int64_t *d : c->d_op_chunk_begin) (void)hipFree(d);
+  for (int32_t *d : c->d_op_chunk_sys) (void)hipFree(d);
+  if (c->ev) (void)hipEventDestroy(c->ev);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   *cg = nullptr;
   return CEED_MI355_SUCCESS;

@@ -404,7 +404,7 @@ extern "C" int CeedMi355BatchSmallDestroy(CeedMi355BatchSmall *bs) {
   for (void *d : {(void *)c->d_systems, (void *)c->d_groups, (void *)c->d_items, (void *)c->d_elem_list, (void *)c->d_color_start,
                   (void *)c->d_counter, (void *)c->d_iters, (void *)c->d_res, (void *)c->r, (void *)c->p, (void *)c->Ap, (void *)c->d_t_start,
                   (void *)c->d_t_done})
-    if (d) hipFree(d);
+    if (d) (void)hipFree(d);
   delete c;
   *bs = nullptr;
   return CEED_MI355_SUCCESS;

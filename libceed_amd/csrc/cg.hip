@@ -280,7 +280,7 @@ int segment(CeedMi355CG cg, const double *b, double *x, int len, bool update) {
 int run_segment(CeedMi355CG cg, const double *b, double *x, int len, bool update, bool use_graph) {
   if (!use_graph) return segment(cg, b, x, len, update);
   if (!cg->graph || cg->graph_len != len || cg->graph_b != b || cg->graph_x != x) {
-    if (cg->graph) hipGraphExecDestroy(cg->graph);
+    if (cg->graph) (void)hipGraphExecDestroy(cg->graph);
     cg->graph = nullptr;
     hipGraph_t g;
     MI355_HIP(hipStreamBeginCapture(cg->stream, hipStreamCaptureModeThreadLocal));
@@ -289,7 +289,7 @@ int run_segment(CeedMi355CG cg, const double *b, double *x, int len, bool update
     if (err) return err;
     MI355_HIP(e);
     e = hipGraphInstantiate(&cg->graph, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     MI355_HIP(e);
     cg->graph_len = len;
     cg->graph_b   = b;
@@ -340,7 +340,7 @@ extern "C" int CeedMi355CGCreate(CeedMi355Operator op, CeedMi355Operator op64, C
 
 extern "C" int CeedMi355CGSetJacobi(CeedMi355CG cg, const double *diag, void *stream) {
   MI355_CHECK(cg, CEED_MI355_ERROR_MAJOR, "null CG");
-  if (cg->graph) hipGraphExecDestroy(cg->graph);  // captured with the other recurrences
+  if (cg->graph) (void)hipGraphExecDestroy(cg->graph);  // captured with the other recurrences
   cg->graph = nullptr;
   if (!diag) {
     cg->jacobi = false;
@@ -422,22 +422,22 @@ extern "C" int CeedMi355CGSolve(CeedMi355CG cg, const double *b, double *x, int3
 extern "C" int CeedMi355CGDestroy(CeedMi355CG *cg) {
   if (!cg || !*cg) return CEED_MI355_SUCCESS;
   CeedMi355CG c = *cg;
-  if (c->stream) hipStreamSynchronize(c->stream);
-  if (c->graph) hipGraphExecDestroy(c->graph);
-  hipFree(c->scal);
-  hipHostFree(c->h_scal);
-  hipFree(c->partial);
-  hipFree(c->partial2);
-  hipFree(c->invd);
-  hipFree(c->z);
-  hipFree(c->r);
-  hipFree(c->p);
-  hipFree(c->Ap);
-  hipFree(c->dx);
-  hipFree(c->r64);
-  hipFree(c->Ap64);
-  if (c->ev) hipEventDestroy(c->ev);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  (void)hipFree(c->scal);
+  (void)hipHostFree(c->h_scal);
+  (void)hipFree(c->partial);
+  (void)hipFree(c->partial2);
+  (void)hipFree(c->invd);
+  (void)hipFree(c->z);
+  (void)hipFree(c->r);
+  (void)hipFree(c->p);
+  (void)hipFree(c->Ap);
+  (void)hipFree(c->dx);
+  (void)hipFree(c->r64);
+  (void)hipFree(c->Ap64);
+  if (c->ev) (void)hipEventDestroy(c->ev);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   *cg = nullptr;
   return CEED_MI355_SUCCESS;

@@ -735,8 +735,8 @@ extern "C" int CeedMi355OperatorBuildQData(CeedMi355Operator op, int build, int3
     ierr = dispatch_build<double>(Q, build, op->num_elem, PM, d_mo, mesh_comp_stride, coords, d_mats, blk, (double *)op->d_qdata, s);
   else ierr = dispatch_build<float>(Q, build, op->num_elem, PM, d_mo, mesh_comp_stride, coords, d_mats, blk, (float *)op->d_qdata, s);
   MI355_HIP(hipStreamSynchronize(s));
-  hipFree(d_mo);
-  hipFree(d_mats);
+  (void)hipFree(d_mo);
+  (void)hipFree(d_mats);
   if (ierr) return ierr;
   if (op->qf == kQfDiffusionKappa) {
     // kappa slot defaults to 1 until CeedMi355OperatorSetKappa
@@ -886,13 +886,14 @@ extern "C" int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int
     op->epb    = epb;
     int e      = CeedMi355OperatorApply(op, u, v, s);
     if (e) return e;
-    hipEventRecord(t0, s);
+    MI355_HIP(hipEventRecord(t0, s));
     for (int r = 0; r < n && !e; r++) e = CeedMi355OperatorApply(op, u, v, s);
-    hipEventRecord(t1, s);
-    hipEventSynchronize(t1);
-    hipEventElapsedTime(ms, t0, t1);
+    if (e) return e;
+    MI355_HIP(hipEventRecord(t1, s));
+    MI355_HIP(hipEventSynchronize(t1));
+    MI355_HIP(hipEventElapsedTime(ms, t0, t1));
     *ms /= n;
-    return e;
+    return CEED_MI355_SUCCESS;
   };
   struct Cand {
     float ms;
@@ -923,10 +924,10 @@ extern "C" int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int
     if (c.ms < bms) bms = c.ms, best = c.epb, best_layout = c.layout;
   op->epb    = err ? start : best;
   op->layout = err ? start_layout : best_layout;
-  hipEventDestroy(t0);
-  hipEventDestroy(t1);
-  hipFree(u);
-  hipFree(v);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  (void)hipFree(u);
+  (void)hipFree(v);
   if (err) return err;
   {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
@@ -954,23 +955,23 @@ extern "C" int CeedMi355OperatorGetColors(CeedMi355Operator op, int32_t *num_col
 extern "C" int CeedMi355OperatorDestroy(CeedMi355Operator *op) {
   if (!op || !*op) return CEED_MI355_SUCCESS;
   if ((*op)->jit) mi355_jit_kernel_destroy((*op)->jit);
-  hipFree((*op)->d_offsets);
-  hipFree((*op)->d_perm);
-  hipFree((*op)->d_mats);
-  hipFree((*op)->d_qdata);
-  hipFree((*op)->d_trow);
-  hipFree((*op)->d_tpos);
-  hipFree((*op)->d_evec);
-  hipFree((*op)->d_srows);
-  hipFree((*op)->d_scratch);
-  hipFree((*op)->d_brick_start);
-  hipFree((*op)->d_brick_mask);
-  hipFree((*op)->d_bnode);
-  hipFree((*op)->d_blist);
-  hipFree((*op)->d_brow);
-  hipFree((*op)->d_bslot);
-  hipFree((*op)->d_partials);
-  hipFree((*op)->d_seg_start);
+  (void)hipFree((*op)->d_offsets);
+  (void)hipFree((*op)->d_perm);
+  (void)hipFree((*op)->d_mats);
+  (void)hipFree((*op)->d_qdata);
+  (void)hipFree((*op)->d_trow);
+  (void)hipFree((*op)->d_tpos);
+  (void)hipFree((*op)->d_evec);
+  (void)hipFree((*op)->d_srows);
+  (void)hipFree((*op)->d_scratch);
+  (void)hipFree((*op)->d_brick_start);
+  (void)hipFree((*op)->d_brick_mask);
+  (void)hipFree((*op)->d_bnode);
+  (void)hipFree((*op)->d_blist);
+  (void)hipFree((*op)->d_brow);
+  (void)hipFree((*op)->d_bslot);
+  (void)hipFree((*op)->d_partials);
+  (void)hipFree((*op)->d_seg_start);
   delete *op;
   *op = nullptr;
   return CEED_MI355_SUCCESS;

@@ -32,7 +32,7 @@ struct CeedMi355JitKernel {
 
 void mi355_jit_kernel_destroy(CeedMi355JitKernel *k) {
   if (!k) return;
-  if (k->module) hipModuleUnload(k->module);
+  if (k->module) (void)hipModuleUnload(k->module);
   delete k;
 }
 

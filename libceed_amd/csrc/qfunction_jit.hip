@@ -120,7 +120,7 @@ int mi355_jit_build(const std::string &code, const char *what, const char *kerne
   hiprtcDestroyProgram(&prog);
   *module = nullptr;
   if (hipModuleLoadData(module, binary.data()) != hipSuccess || hipModuleGetFunction(fn, *module, kernel_name) != hipSuccess) {
-    if (*module) hipModuleUnload(*module);
+    if (*module) (void)hipModuleUnload(*module);
     *module = nullptr;
     MI355_CHECK(false, CEED_MI355_ERROR_BACKEND, "loading the compiled kernel %s failed", what);
   }
@@ -208,7 +208,7 @@ extern "C" int CeedMi355QFunctionApply(CeedMi355QFunction qf, int64_t num_elem, 
 
 extern "C" int CeedMi355QFunctionDestroy(CeedMi355QFunction *qf) {
   if (!qf || !*qf) return CEED_MI355_SUCCESS;
-  if ((*qf)->module) hipModuleUnload((*qf)->module);
+  if ((*qf)->module) (void)hipModuleUnload((*qf)->module);
   delete *qf;
   *qf = nullptr;
   return CEED_MI355_SUCCESS;

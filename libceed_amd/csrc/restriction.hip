@@ -177,10 +177,10 @@ extern "C" int CeedMi355RestrictionApply(CeedMi355Restriction r, int t_mode, con
 extern "C" int CeedMi355RestrictionDestroy(CeedMi355Restriction *rstr) {
   if (!rstr || !*rstr) return CEED_MI355_SUCCESS;
   CeedMi355Restriction r = *rstr;
-  hipFree(r->d_offsets);
-  hipFree(r->d_t_list);
-  hipFree(r->d_t_start);
-  hipFree(r->d_t_index);
+  (void)hipFree(r->d_offsets);
+  (void)hipFree(r->d_t_list);
+  (void)hipFree(r->d_t_start);
+  (void)hipFree(r->d_t_index);
   free(r);
   *rstr = nullptr;
   return CEED_MI355_SUCCESS;
