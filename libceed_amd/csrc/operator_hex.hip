@@ -881,7 +881,7 @@ extern "C" int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int
   float     bms  = 1e30f;
   int       err  = CEED_MI355_SUCCESS;
   // ms per apply of one launch shape: a warm-up apply, then n timed applies
-  auto time_launch = [&](int layout, int epb, int n, float *ms) {
+  auto time_launch = [&](int layout, int epb, int n, float *ms) -> int {
     op->layout = layout;
     op->epb    = epb;
     int e      = CeedMi355OperatorApply(op, u, v, s);
