@@ -21,7 +21,7 @@
 //   x-line of nodes (y, z) = (t % P, t / P), t < P*P     B_x, B_x^T
 //   y-line (qx, z) = (t % Q, t / Q), t < Q*P              B_y, B_y^T
 //   x-line (qy, qz) and y-line (qx, qz) = (t % Q, t / Q)  the collocated derivatives
-// Slabs are [z][y][x] with strides (Q*Q, Q, 1), padded at Q = 8 against LDS
+// Slabs are [z][y][x] with strides (Q*Q, Q, 1), padded at Q = 7, 8 against LDS
 // bank conflicts (line_row_stride, line_plane_stride); mass needs two per element,
 // diffusion three (the y-derivative transpose waits in registers for a barrier
 // and then takes the slab the x-derivative transpose has read: one barrier more
@@ -77,9 +77,11 @@ struct LineSlabs {
 // patterns: 2.1x fewer conflict cycles in FP64, 3.8x in FP32).  Measured on
 // MI355X, 8M DoFs, ms per apply: FP32 p=6 0.136-0.139 vs 0.148, 3-component
 // FP64 p=6 0.648-0.650 vs 0.689-0.691, scalar FP64 p=6 unchanged within run
-// to run noise; the same search gave no measurable gain at Q = 4 or 10.
+// to run noise.  At Q = 7 (FP64) planes of 58 model 10% fewer conflict
+// cycles: 3-component p=5 0.783-0.784 vs 0.794-0.796 ms, scalar p=5
+// unchanged; the same search gave no measurable gain at Q = 4 or 10.
 __host__ __device__ constexpr int line_row_stride(int Q, int sb) { return sb == 8 && Q == 8 ? 9 : Q; }
-__host__ __device__ constexpr int line_plane_stride(int Q, int sb) { return Q == 8 ? (sb == 8 ? 72 : 73) : Q * Q; }
+__host__ __device__ constexpr int line_plane_stride(int Q, int sb) { return Q == 8 ? (sb == 8 ? 72 : 73) : Q == 7 && sb == 8 ? 58 : Q * Q; }
 __host__ __device__ constexpr int line_slab_entries(int Q, int sb) { return Q * line_plane_stride(Q, sb); }
 
 template <int P, int Q, int QF, typename T>
