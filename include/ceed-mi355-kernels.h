@@ -225,7 +225,8 @@ int CeedMi355OperatorSetLayout(CeedMi355Operator op, int32_t layout, int32_t ele
 int CeedMi355OperatorGetLaunch(CeedMi355Operator op, int32_t *layout, int32_t *elems_per_block);
 /* Autotuner (the reference has none: its hip-shared/hip-gen backends fix the
  * launch shape per P, Q): time Apply (reps times) for every admissible
- * (thread layout, elements per workgroup) on scratch vectors, keep the
+ * (thread layout, elements per workgroup) on scratch vectors, time the three
+ * fastest again (twice each, 2 * reps applies, best time kept), keep the
  * fastest in the plan and remember it for later plans of the same (P, Q,
  * num_comp, pointwise operator, scalar type, E^T mode).  best_epb_host gets
  * the elements per workgroup; CeedMi355OperatorGetLaunch gives the layout. */
