@@ -443,6 +443,7 @@ __global__ void __launch_bounds__(256, (lines_min_waves<P, Q, NCOMP, QF, T>())) 
                                                                const int32_t *__restrict__ seg_start) {
   constexpr int NT = Q * Q, P3 = P * P * P, Q3 = Q * Q * Q, QC = QfTraits<QF>::kComps;
   constexpr int SR = line_row_stride(Q, sizeof(T)), SP = line_plane_stride(Q, sizeof(T)), SLAB = Q * SP;
+  static_assert(SR >= Q && SP >= Q * SR, "slab rows and planes must not overlap");
   constexpr int NS = LineSlabs<QF>::kSlabs;
   // from Q = 7 one phase's matrix entries no longer fit the SGPRs: load them
   // per output row (or column) instead of per phase
