@@ -869,17 +869,26 @@ extern "C" int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int
   MI355_CHECK(op && reps >= 1, CEED_MI355_ERROR_MAJOR, "bad autotune arguments");
   hipStream_t s = (hipStream_t)stream;
   void       *u = nullptr, *v = nullptr;
-  hipEvent_t  t0, t1;
-  MI355_HIP(hipMalloc(&u, (size_t)op->scalar_bytes * (op->l_size + 1)));
-  MI355_HIP(hipMalloc(&v, (size_t)op->scalar_bytes * (op->l_size + 1)));
-  MI355_HIP(hipEventCreate(&t0));
-  MI355_HIP(hipEventCreate(&t1));
+  hipEvent_t  t0 = nullptr, t1 = nullptr;
+  auto        release = [&]() {
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    (void)hipFree(u);
+    (void)hipFree(v);
+  };
+  const size_t bytes = (size_t)op->scalar_bytes * (op->l_size + 1);
+  if (hipMalloc(&u, bytes) != hipSuccess || hipMalloc(&v, bytes) != hipSuccess || hipEventCreate(&t0) != hipSuccess ||
+      hipEventCreate(&t1) != hipSuccess) {
+    release();
+    MI355_CHECK(false, CEED_MI355_ERROR_BACKEND, "autotune: scratch vectors or events could not be created (%zu bytes)", bytes);
+  }
   if (op->scalar_type == CEED_MI355_FP64) tune_fill<double><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, (double *)u);
   else tune_fill<float><<<mi355_blocks(op->l_size, 256, 4096), 256, 0, s>>>(op->l_size, (float *)u);
   const int start = op->epb, start_layout = op->layout;
   int       best = start, best_layout = start_layout;
   float     bms  = 1e30f;
-  int       err  = CEED_MI355_SUCCESS;
+  int       err  = hipGetLastError() == hipSuccess ? CEED_MI355_SUCCESS : CEED_MI355_ERROR_BACKEND;
+  if (err) mi355::set_error("autotune: the fill of the scratch input failed to launch");
   // ms per apply of one launch shape: a warm-up apply, then n timed applies
   auto time_launch = [&](int layout, int epb, int n, float *ms) -> int {
     op->layout = layout;
@@ -924,10 +933,7 @@ extern "C" int CeedMi355OperatorAutotune(CeedMi355Operator op, int32_t reps, int
     if (c.ms < bms) bms = c.ms, best = c.epb, best_layout = c.layout;
   op->epb    = err ? start : best;
   op->layout = err ? start_layout : best_layout;
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
-  (void)hipFree(u);
-  (void)hipFree(v);
+  release();
   if (err) return err;
   {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
